@@ -1,0 +1,187 @@
+"""Seeded initial conditions for the benchmark problems, in conserved variables.
+
+Every implementation (HIP path, CPU oracle, reference build) starts from the arrays these
+functions return, so the initial data is bit-identical everywhere.  Primitive -> conserved
+follows EquationOfState::PrimToCons (src/eos/adiabatic.cpp:162-198) operation for operation:
+``E = p*(1/(gamma-1)) + 0.5*rho*(vx*vx + vy*vy + vz*vz)``.
+
+Random perturbations use SplitMix64 (Steele, Lea & Flood 2014) keyed by (seed, cell index),
+so a value depends only on the seed and the cell, never on array order or thread count.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from .gkio import cell_centres
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 output function applied to state ``x`` (uint64, wraps mod 2**64)."""
+    with np.errstate(over="ignore"):
+        z = (x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)) & _M64
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_pm1(seed: int, idx: np.ndarray, stream: int = 0) -> np.ndarray:
+    """Uniform in [-1, 1): 53 high bits of SplitMix64(seed*2**32 + stream*2**28 + idx)."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) << np.uint64(32)) + (np.uint64(stream) << np.uint64(28)) + idx.astype(np.uint64)
+    u = (splitmix64(key) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 2.0 * u - 1.0
+
+
+@dataclass
+class Problem:
+    """Grid, physics and boundary parameters plus the active-region conserved state."""
+    name: str
+    nx: int
+    ny: int
+    extent: tuple            # (xmin, xmax, ymin, ymax, zmin, zmax)
+    gamma: float = 1.4
+    cfl: float = 0.4
+    integrator: int = 2      # 1 Euler, 2 RK2, 3 RK3
+    rc_order: int = 2
+    stepmax: int = -1
+    tmax: float = 1e30
+    dtoutput: float = 1e30
+    bc: tuple = (1, 1, 1, 1)  # xleft, xright, yleft, yright
+    g: tuple = (0.0, 0.0, 0.0)
+    ng: int = 2
+    cons: np.ndarray = field(default=None, repr=False)  # (5, ny, nx) float64
+
+    def with_(self, **kw) -> "Problem":
+        return replace(self, **kw)
+
+    def in_values(self) -> dict:
+        """Keys of the reference's `.in` grammar describing this problem."""
+        x0, x1, y0, y1, z0, z1 = self.extent
+        return {
+            "gamma": self.gamma, "tmax": self.tmax, "stepmax": self.stepmax, "CFL": self.cfl,
+            "integrator": self.integrator, "RcOrder": self.rc_order, "dtoutput": self.dtoutput,
+            "dataType": 0, "xmin": x0, "xmax": x1, "ymin": y0, "ymax": y1, "zmin": z0, "zmax": z1,
+            "NX": self.nx, "NY": self.ny, "NZ": 1, "NGhost": self.ng,
+            "xleft": self.bc[0], "xright": self.bc[1], "yleft": self.bc[2], "yright": self.bc[3],
+            "zleft": 1, "zright": 1, "gx": self.g[0], "gy": self.g[1], "gz": self.g[2],
+        }
+
+
+def prim_to_cons(rho, vx, vy, vz, p, gamma: float) -> np.ndarray:
+    gm1rec = 1.0 / (gamma - 1.0)
+    e = p * gm1rec + (0.5 * rho) * (vx * vx + vy * vy + vz * vz)
+    return np.stack([rho, rho * vx, rho * vy, rho * vz, e]).astype(np.float64)
+
+
+def _centres(extent, nx, ny, ng):
+    x = cell_centres(extent[0], extent[1], nx, ng)[ng:ng + nx]
+    y = cell_centres(extent[2], extent[3], ny, ng)[ng:ng + ny]
+    return np.meshgrid(x, y, indexing="xy")  # arrays of shape (ny, nx)
+
+
+def sod(nx: int = 256, **kw) -> Problem:
+    """Sod shock tube on [0,1]: (rho, p) = (1, 1) for x < 0.5 else (0.125, 0.1), v = 0.
+
+    Setup logic of src/setup/setup_shock_tube.cpp:10-48 with x0 = 0.5; y/z extents of 1 keep
+    drmin = dx (src/sim.cpp:35)."""
+    ext = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    x, _ = _centres(ext, nx, 1, 2)
+    left = x < 0.5
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    z = np.zeros_like(rho)
+    base = dict(name="sod", nx=nx, ny=1, extent=ext, integrator=2, rc_order=2, cfl=0.4,
+                tmax=0.2, dtoutput=0.2, bc=(0, 0, 0, 0))
+    base.update(kw)
+    return Problem(cons=prim_to_cons(rho, z, z, z, p, base.get("gamma", 1.4)), **base)
+
+
+def kelvin_helmholtz(n: int = 2048, ny: int | None = None, seed: int = 0, amp: float = 0.01,
+                     noise: float = 0.01, **kw) -> Problem:
+    """Periodic KH on [0,1]^2: rho = 2, vx = +0.5 for |y-0.5| < 0.25, else rho = 1, vx = -0.5;
+    p = 2.5; vy = amp*sin(4 pi x) + noise*U(-1,1) (SplitMix64 keyed by j*nx + i)."""
+    ny = n if ny is None else ny
+    ext = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    x, y = _centres(ext, n, ny, 2)
+    inner = np.abs(y - 0.5) < 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    vx = np.where(inner, 0.5, -0.5)
+    idx = np.arange(n * ny, dtype=np.uint64).reshape(ny, n)
+    vy = amp * np.sin(4.0 * np.pi * x) + noise * uniform_pm1(seed, idx)
+    p = np.full_like(rho, 2.5)
+    base = dict(name="kh", nx=n, ny=ny, extent=ext, integrator=3, rc_order=2, cfl=0.4,
+                stepmax=200, bc=(1, 1, 1, 1))
+    base.update(kw)
+    return Problem(cons=prim_to_cons(rho, vx, vy, np.zeros_like(rho), p, base.get("gamma", 1.4)), **base)
+
+
+def rayleigh_taylor(nx: int = 1024, ny: int = 3072, gy: float = -0.1, p0: float = 2.5,
+                    amp: float = 0.01, **kw) -> Problem:
+    """RT on [0,0.5]x[0,1.5], rho = 2 above y = 0.75 else 1, hydrostatic
+    p = p0 + gy*rho*(y - 0.75) (the form of src/setup/setup_rt.cpp:54 with the interface at
+    0.75), vy = amp*(1+cos 4 pi x)(1+cos 3 pi (y-0.75))/4; periodic x, reflecting y, gravity gy."""
+    ext = (0.0, 0.5, 0.0, 1.5, 0.0, 1.0)
+    x, y = _centres(ext, nx, ny, 2)
+    rho = np.where(y > 0.75, 2.0, 1.0)
+    p = p0 + gy * rho * (y - 0.75)
+    vy = amp * (1.0 + np.cos(4.0 * np.pi * x)) * (1.0 + np.cos(3.0 * np.pi * (y - 0.75))) / 4.0
+    z = np.zeros_like(rho)
+    base = dict(name="rt", nx=nx, ny=ny, extent=ext, integrator=2, rc_order=2, cfl=0.4,
+                bc=(1, 1, 2, 2), g=(0.0, gy, 0.0))
+    base.update(kw)
+    return Problem(cons=prim_to_cons(rho, z, vy, z, p, base.get("gamma", 1.4)), **base)
+
+
+def blast(nx: int, ny: int, p_ratio: float = 50.0, radius: float = 0.1, center=None,
+          bc: int = 1, rho0: float = 1.0, p_amb: float = 0.1, **kw) -> Problem:
+    """2D circular overpressure: rho = rho0, p = p_amb*p_ratio inside r < radius, else p_amb,
+    v = 0, on [0, nx/L] x [0, ny/L] with L = max(nx, ny) so dx = dy = 1/L."""
+    L = float(max(nx, ny))
+    ext = (0.0, nx / L, 0.0, ny / L, 0.0, 1.0)
+    cx, cy = center if center is not None else (0.5 * nx / L, 0.5 * ny / L)
+    x, y = _centres(ext, nx, ny, 2)
+    r = np.sqrt((x - cx) ** 2 + (y - cy) ** 2)
+    rho = np.full_like(x, rho0)
+    p = np.where(r < radius, p_amb * p_ratio, p_amb)
+    z = np.zeros_like(rho)
+    bcs = (bc,) * 4 if isinstance(bc, int) else tuple(bc)
+    base = dict(name="blast", nx=nx, ny=ny, extent=ext, integrator=2, rc_order=2, cfl=0.4, bc=bcs)
+    base.update(kw)
+    return Problem(cons=prim_to_cons(rho, z, z, z, p, base.get("gamma", 1.4)), **base)
+
+
+def riemann_1d(nx: int, left, right, x0: float = 0.5, **kw) -> Problem:
+    """1D Riemann problem on [0,1]: left/right = (rho, vx, p)."""
+    ext = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    x, _ = _centres(ext, nx, 1, 2)
+    L = x < x0
+    rho = np.where(L, left[0], right[0])
+    vx = np.where(L, left[1], right[1])
+    p = np.where(L, left[2], right[2])
+    z = np.zeros_like(rho)
+    base = dict(name="riemann", nx=nx, ny=1, extent=ext, integrator=2, rc_order=2, cfl=0.4, bc=(0, 0, 0, 0))
+    base.update(kw)
+    return Problem(cons=prim_to_cons(rho, vx, z, z, p, base.get("gamma", 1.4)), **base)
+
+
+def ensemble(count: int = 512, seed: int = 0, nmin: int = 64, nmax: int = 512) -> list:
+    """Seeded ensemble of 2D blast problems: nx, ny ~ U{nmin..nmax} independently; p ratio
+    U(10, 100); radius U(0.05, 0.2); centre at the domain centre; BC periodic or outflow by a
+    seeded coin.  All FP-free choices come from SplitMix64 streams of ``seed``."""
+    k = np.arange(count, dtype=np.uint64)
+    span = nmax - nmin + 1
+    nxs = nmin + (splitmix64((np.uint64(seed) << np.uint64(32)) + k * np.uint64(8) + np.uint64(1)) % np.uint64(span)).astype(np.int64)
+    nys = nmin + (splitmix64((np.uint64(seed) << np.uint64(32)) + k * np.uint64(8) + np.uint64(2)) % np.uint64(span)).astype(np.int64)
+    ratio = 10.0 + 90.0 * (0.5 * (uniform_pm1(seed, k, stream=3) + 1.0))
+    rad = 0.05 + 0.15 * (0.5 * (uniform_pm1(seed, k, stream=4) + 1.0))
+    coin = (splitmix64((np.uint64(seed) << np.uint64(32)) + k * np.uint64(8) + np.uint64(5)) & np.uint64(1)).astype(np.int64)
+    out = []
+    for q in range(count):
+        bc = 1 if coin[q] == 0 else 3
+        out.append(blast(int(nxs[q]), int(nys[q]), p_ratio=float(ratio[q]), radius=float(rad[q]), bc=bc,
+                         name=f"blast{q}"))
+    return out

@@ -1,0 +1,530 @@
+// gko_oracle.cpp -- CPU restatement of Gaukuk's hydro step (TEST INFRASTRUCTURE ONLY).
+//
+// Every function below cites the reference file:line whose arithmetic it restates.  The
+// order of every floating-point operation is kept, because the strict build of this file
+// must be bitwise equal to the strict build of the reference.  Only tests, smoke() and the
+// bench's CPU-baseline leg use this library; the HIP product path never does.
+#include "gko_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+namespace {
+
+constexpr int NV = 5;
+enum { DEN = 0, MX = 1, MY = 2, MZ = 3, ENG = 4 };
+enum { VX = 1, VY = 2, VZ = 3, PRE = 4 };
+
+// std::min / std::max semantics (first argument returned on ties / unordered operands).
+template <class R> inline R smin(R a, R b) { return (b < a) ? b : a; }
+template <class R> inline R smax(R a, R b) { return (a < b) ? b : a; }
+
+template <class Real>
+struct Oracle {
+    gko_params P;
+    int nx, ny, ng, lenx, leny, ib, ie, jb, je;
+    Real gamma, gm1Rec, cfl;
+    Real dx, dy, dz, dxRec, dyRec, drmin;
+    Real gx, gy, gz;
+    bool src_enrolled;
+    // state
+    std::vector<Real> cons, prim, flx1, flx2, consTemp;
+    Real t, dt, dtUntilOutput, cmax;
+    int step;
+    bool isContinue;
+    std::vector<double> log_t, log_dt;
+
+    explicit Oracle(const gko_params& p) : P(p) {
+        nx = p.nx; ny = p.ny; ng = p.ng;
+        if (nx <= 0 || ny <= 0 || ng < 0) throw std::runtime_error("Invalid grid parameters");
+        for (int f = 0; f < 4; f++)
+            if (p.bc[f] == 4) throw std::runtime_error("fixed boundary (type 4) is not restated");
+        lenx = nx + 2 * ng; leny = ny + 2 * ng;
+        ib = ng; ie = ib + nx; jb = ng; je = jb + ny;
+        // EquationOfState ctor: src/eos/adiabatic.cpp:12-16
+        gamma = (Real)p.gamma;
+        gm1Rec = Real(1.0) / (gamma - Real(1.0));
+        cfl = (Real)p.cfl;
+        // Domain ctor: src/sim.cpp:16-36 (nz == 1)
+        dx = ((Real)p.xmax - (Real)p.xmin) / (Real)nx;
+        dy = ((Real)p.ymax - (Real)p.ymin) / (Real)ny;
+        dz = ((Real)p.zmax - (Real)p.zmin) / (Real)1;
+        dxRec = Real(1.) / dx;
+        dyRec = Real(1.) / dy;
+        if (ny <= 1) dyRec = 0;
+        drmin = smin(smin(dx, dy), dz);
+        // SourceTerm::EnrollConstGravityVector: src/source_term/source.cpp:23-31
+        gx = (Real)p.g[0]; gy = (Real)p.g[1]; gz = (Real)p.g[2];
+        src_enrolled = (gx != 0 || gy != 0 || gz != 0);
+        size_t n = (size_t)NV * leny * lenx;
+        cons.assign(n, 0); prim.assign(n, 0); consTemp.assign(n, 0);
+        flx1.assign((size_t)NV * leny * (lenx + 1), 0);
+        flx2.assign((size_t)NV * (leny + 1) * lenx, 0);
+        // Sim ctor: src/sim.cpp:45-58
+        t = 0; dt = (Real)1e10; dtUntilOutput = (Real)1e10; cmax = (Real)1e-16;
+        step = 0; isContinue = true;
+    }
+
+    inline size_t at(int n, int j, int i) const { return (size_t)i + (size_t)lenx * ((size_t)j + (size_t)leny * n); }
+    inline size_t at1(int n, int j, int i) const { return (size_t)i + (size_t)(lenx + 1) * ((size_t)j + (size_t)leny * n); }
+    inline size_t at2(int n, int j, int i) const { return (size_t)i + (size_t)lenx * ((size_t)j + (size_t)(leny + 1) * n); }
+
+    // ---- EOS -------------------------------------------------------------------------
+    // ConsToPrim over all cells incl. ghosts: src/eos/adiabatic.cpp:18-61
+    void ConsToPrim(const std::vector<Real>& U, std::vector<Real>& W) {
+        const Real gm1 = gamma - Real(1.0);
+        const Real dmin = (Real)1e-16, pmin = (Real)1e-16;
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < leny; j++) {
+            for (int i = 0; i < lenx; i++) {
+                Real d = U[at(DEN, j, i)], m1 = U[at(MX, j, i)], m2 = U[at(MY, j, i)],
+                     m3 = U[at(MZ, j, i)], e = U[at(ENG, j, i)];
+                d = smax(d, dmin);
+                W[at(DEN, j, i)] = d;
+                Real inv = Real(1.0) / d;
+                W[at(VX, j, i)] = m1 * inv;
+                W[at(VY, j, i)] = m2 * inv;
+                W[at(VZ, j, i)] = m3 * inv;
+                Real ek = Real(0.5) * inv * (m1 * m1 + m2 * m2 + m3 * m3);
+                Real p = gm1 * (e - ek);
+                W[at(PRE, j, i)] = (p > pmin) ? p : pmin;
+            }
+        }
+    }
+
+    // CalCmax over active cells: src/eos/adiabatic.cpp:115-160 (its own kinetic-energy form, :148)
+    void CalCmax(const std::vector<Real>& U, Real& cm) {
+        const Real gm1 = gamma - Real(1.0);
+        const Real dmin = (Real)1e-16, pmin = (Real)1e-16;
+        Real best = cm;
+#pragma omp parallel for schedule(static) reduction(max : best)
+        for (int j = jb; j < je; j++) {
+            for (int i = ib; i < ie; i++) {
+                Real d = U[at(DEN, j, i)], m1 = U[at(MX, j, i)], m2 = U[at(MY, j, i)],
+                     m3 = U[at(MZ, j, i)], e = U[at(ENG, j, i)];
+                d = smax(d, dmin);
+                Real inv = Real(1.0) / d;
+                Real v1 = m1 * inv, v2 = m2 * inv, v3 = m3 * inv;
+                Real ek = Real(0.5) * (v1 * m1 + v2 * m2 + v3 * m3);
+                Real p = gm1 * (e - ek);
+                p = (p > pmin) ? p : pmin;
+                Real cs = std::sqrt(gamma * p * inv);
+                Real loc = std::abs(v1) + cs;
+                loc = smax(loc, std::abs(v2) + cs);
+                loc = smax(loc, std::abs(v3) + cs);
+                best = smax(best, loc);
+            }
+        }
+        cm = best;
+    }
+
+    // ---- Boundaries (types 0..3), each face over the active tangential range ------------
+    // Type 0: src/boundary/outflow_copy.cpp:16-124; 1: periodic.cpp:19-131;
+    // 2: reflective.cpp:19-166; 3: outflow.cpp:20-213.  Dispatch order Bdxl, Bdxr, Bdyl,
+    // Bdyr: src/boundary/boundary.cpp:127-142 (z faces only when nz > 1: not here).
+    // Unknown codes fall back to type 0 (boundary.cpp:27-29 and siblings).
+    void FillX(std::vector<Real>& U, bool left, int type) {
+        const int gl = left ? 0 : ie, gr = left ? ib : lenx;
+        for (int j = jb; j < je; j++) {
+            for (int i = gl; i < gr; i++) {
+                if (type == 1) {
+                    int src = left ? i + nx : i - nx;
+                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, j, src)];
+                } else if (type == 2 || type == 3) {
+                    int src = left ? (ib + ib - i - 1) : (ie + ie - i - 1);
+                    Real d = U[at(DEN, j, src)], m1 = U[at(MX, j, src)], m2 = U[at(MY, j, src)],
+                         m3 = U[at(MZ, j, src)], e = U[at(ENG, j, src)];
+                    if (type == 2) {
+                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = -m1; U[at(MY, j, i)] = m2;
+                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = e;
+                    } else {
+                        Real mc = left ? smin(m1, Real(0.0)) : smax(m1, Real(0.0));
+                        d = smax(d, (Real)1e-16);
+                        Real inv = Real(1.0) / d;
+                        Real keOld = Real(0.5) * inv * m1 * m1;
+                        Real keNew = Real(0.5) * inv * mc * mc;
+                        Real eint = e - keOld;
+                        eint = smax(eint, (Real)1e-16);
+                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = mc; U[at(MY, j, i)] = m2;
+                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = eint + keNew;
+                    }
+                } else {
+                    int src = left ? ib : ie - 1;
+                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, j, src)];
+                }
+            }
+        }
+    }
+    void FillY(std::vector<Real>& U, bool left, int type) {
+        const int gl = left ? 0 : je, gr = left ? jb : leny;
+        for (int j = gl; j < gr; j++) {
+            for (int i = ib; i < ie; i++) {
+                if (type == 1) {
+                    int src = left ? j + ny : j - ny;
+                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, src, i)];
+                } else if (type == 2 || type == 3) {
+                    int src = left ? (jb + jb - j - 1) : (je + je - j - 1);
+                    Real d = U[at(DEN, src, i)], m1 = U[at(MX, src, i)], m2 = U[at(MY, src, i)],
+                         m3 = U[at(MZ, src, i)], e = U[at(ENG, src, i)];
+                    if (type == 2) {
+                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = m1; U[at(MY, j, i)] = -m2;
+                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = e;
+                    } else {
+                        Real mc = left ? smin(m2, Real(0.0)) : smax(m2, Real(0.0));
+                        d = smax(d, (Real)1e-16);
+                        Real inv = Real(1.0) / d;
+                        Real keOld = Real(0.5) * inv * m2 * m2;
+                        Real keNew = Real(0.5) * inv * mc * mc;
+                        Real eint = e - keOld;
+                        eint = smax(eint, (Real)1e-16);
+                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = m1; U[at(MY, j, i)] = mc;
+                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = eint + keNew;
+                    }
+                } else {
+                    int src = left ? jb : je - 1;
+                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, src, i)];
+                }
+            }
+        }
+    }
+    void UpdateBD(std::vector<Real>& U) {
+        FillX(U, true, P.bc[0]);
+        FillX(U, false, P.bc[1]);
+        FillY(U, true, P.bc[2]);
+        FillY(U, false, P.bc[3]);
+    }
+
+    // ---- Reconstruction + HLLC ---------------------------------------------------------
+    // VanLeer: src/reconstruction/reconstruction.hpp:36-42 (division evaluated, then discarded)
+    static inline Real VanLeer(Real a, Real b) {
+        Real multi = a * b;
+        Real sigma = Real(2.0) * multi / (a + b);
+        return (multi <= Real(0.0)) ? Real(0.0) : sigma;
+    }
+
+    // HLLC at one face: src/flux/hllc.cpp:22-103.  wl/wr are primitive states (den, v_n,
+    // v_t1, v_t2, p) already rotated so that index 1 is the face normal; f receives
+    // (F_den, F_n, F_t1, F_t2, F_E) in the same rotated order.
+    inline void Hllc(const Real* wl, const Real* wr, Real* f) const {
+        const Real gmRec = Real(1.0) / gamma;
+        Real denl = wl[0], vxl = wl[1], vyl = wl[2], vzl = wl[3], prel = wl[4];
+        Real denr = wr[0], vxr = wr[1], vyr = wr[2], vzr = wr[3], prer = wr[4];
+        Real al = std::sqrt(gamma * prel / denl);
+        Real ar = std::sqrt(gamma * prer / denr);
+        Real pv = Real(0.5) * (prel + prer) - Real(0.125) * (vxr - vxl) * (denl + denr) * (al + ar);
+        pv = smax(pv, Real(0));
+        Real tmp = smax(pv / prel - 1, Real(0));
+        Real ql = std::sqrt(1 + Real(0.5) * (1 + gmRec) * tmp);
+        tmp = smax(pv / prer - 1, Real(0));
+        Real qr = std::sqrt(1 + Real(0.5) * (1 + gmRec) * tmp);
+        Real sl = vxl - al * ql;
+        Real sr = vxr + ar * qr;
+        Real ss = (prer - prel + denl * vxl * (sl - vxl) - denr * vxr * (sr - vxr)) /
+                  (denl * (sl - vxl) - denr * (sr - vxr));
+        Real el = prel * gm1Rec + Real(0.5) * denl * (vxl * vxl + vyl * vyl + vzl * vzl);
+        Real er = prer * gm1Rec + Real(0.5) * denr * (vxr * vxr + vyr * vyr + vzr * vzr);
+        Real cl = smin(Real(0), sl) * (ss - vxl) / (sl - ss);
+        Real cr = smax(Real(0), sr) * (ss - vxr) / (sr - ss);
+        Real sel = (ss >= 0) ? Real(1.0) : Real(0.0);
+        Real ser = Real(1.0) - sel;
+        Real fl1 = denl * (vxl + cl);
+        Real fl2 = denl * (vxl * vxl + cl * sl) + prel;
+        Real fl3 = fl1 * vyl;
+        Real fl4 = fl1 * vzl;
+        Real fl5 = (el + prel) * (vxl + cl) + denl * cl * ss * (sl - vxl);
+        Real fr1 = denr * (vxr + cr);
+        Real fr2 = denr * (vxr * vxr + cr * sr) + prer;
+        Real fr3 = fr1 * vyr;
+        Real fr4 = fr1 * vzr;
+        Real fr5 = (er + prer) * (vxr + cr) + denr * cr * ss * (sr - vxr);
+        f[0] = sel * fl1 + ser * fr1;
+        f[1] = sel * fl2 + ser * fr2;
+        f[2] = sel * fl3 + ser * fr3;
+        f[3] = sel * fl4 + ser * fr4;
+        f[4] = sel * fl5 + ser * fr5;
+    }
+
+    // Left state at face f from cell f-1, right state from cell f (PLM: recon_piecewise_linear.cpp:8-55;
+    // first order: recon_first_order.cpp:8-29).  dir 1 = x (cells along i), 2 = y (cells along j).
+    inline Real CellVal(int n, int j, int i) const { return prim[at(n, j, i)]; }
+
+    // x sweep over active rows, faces ib..ie (the faces UpdateCons reads): cal_flux.cpp:90-101
+    void SweepX() {
+        const int rotate[NV] = {DEN, VX, VY, VZ, PRE};
+#pragma omp parallel for schedule(static)
+        for (int j = jb; j < je; j++) {
+            Real wl[NV], wr[NV], f[NV];
+            for (int fi = ib; fi <= ie; fi++) {
+                for (int q = 0; q < NV; q++) {
+                    int n = rotate[q];
+                    if (P.rc_order == 1) {
+                        wl[q] = CellVal(n, j, fi - 1);
+                        wr[q] = CellVal(n, j, fi);
+                    } else {
+                        Real a0 = CellVal(n, j, fi - 2), a1 = CellVal(n, j, fi - 1),
+                             a2 = CellVal(n, j, fi), a3 = CellVal(n, j, fi + 1);
+                        Real sl = VanLeer(a1 - a0, a2 - a1);
+                        Real sr = VanLeer(a2 - a1, a3 - a2);
+                        wl[q] = a1 + Real(0.5) * sl;
+                        wr[q] = a2 - Real(0.5) * sr;
+                    }
+                }
+                Hllc(wl, wr, f);
+                for (int q = 0; q < NV; q++) flx1[at1(rotate[q], j, fi)] = f[q];
+            }
+        }
+    }
+
+    // y sweep, faces jb..je over active columns: cal_flux.cpp:103-121.  HLLC rotation for
+    // direction VLY (hllc.cpp:25-27): normal vy, tangents (vz, vx).
+    void SweepY() {
+        const int rotate[NV] = {DEN, VY, VZ, VX, PRE};
+#pragma omp parallel for schedule(static)
+        for (int fj = jb; fj <= je; fj++) {
+            Real wl[NV], wr[NV], f[NV];
+            for (int i = ib; i < ie; i++) {
+                for (int q = 0; q < NV; q++) {
+                    int n = rotate[q];
+                    if (P.rc_order == 1) {
+                        wl[q] = CellVal(n, fj - 1, i);
+                        wr[q] = CellVal(n, fj, i);
+                    } else {
+                        Real a0 = CellVal(n, fj - 2, i), a1 = CellVal(n, fj - 1, i),
+                             a2 = CellVal(n, fj, i), a3 = CellVal(n, fj + 1, i);
+                        Real sl = VanLeer(a1 - a0, a2 - a1);
+                        Real sr = VanLeer(a2 - a1, a3 - a2);
+                        wl[q] = a1 + Real(0.5) * sl;
+                        wr[q] = a2 - Real(0.5) * sr;
+                    }
+                }
+                Hllc(wl, wr, f);
+                for (int q = 0; q < NV; q++) flx2[at2(rotate[q], fj, i)] = f[q];
+            }
+        }
+    }
+
+    void CalFlux() { SweepX(); SweepY(); }
+
+    // ---- UpdateCons: src/evolution/update_cons.cpp:11-75 (2 coef), 135-205 (3 coef) ---------
+    void UpdateCons(const std::vector<Real>& U0, std::vector<Real>& UT, Real c1, Real c2, Real c3, bool three) {
+        const Real dtdx = dt * dxRec, dtdy = dt * dyRec;
+        const Real dmin = (Real)1e-16, pmin = (Real)1e-16;
+        const Real gm1 = gamma - 1;
+#pragma omp parallel for schedule(static)
+        for (int j = jb; j < je; j++) {
+            for (int i = ib; i < ie; i++) {
+                Real r[NV];
+                for (int n = 0; n < NV; n++) {
+                    const Real dF = (flx1[at1(n, j, i + 1)] - flx1[at1(n, j, i)]) * dtdx +
+                                    (flx2[at2(n, j + 1, i)] - flx2[at2(n, j, i)]) * dtdy;
+                    if (three)
+                        r[n] = c1 * U0[at(n, j, i)] + c2 * UT[at(n, j, i)] - c3 * dF;
+                    else
+                        r[n] = c1 * U0[at(n, j, i)] - c2 * dF;
+                }
+                if (r[DEN] < dmin) {
+                    r[DEN] = dmin; r[MX] = 0.0; r[MY] = 0.0; r[MZ] = 0.0;
+                    r[ENG] = pmin / gm1 + Real(0.0);
+                } else {
+                    Real inv = Real(1.0) / r[DEN];
+                    Real ke = Real(0.5) * inv * (r[MX] * r[MX] + r[MY] * r[MY] + r[MZ] * r[MZ]);
+                    Real press = gm1 * (r[ENG] - ke);
+                    if (press < pmin) r[ENG] = pmin / gm1 + ke;
+                }
+                for (int n = 0; n < NV; n++) UT[at(n, j, i)] = r[n];
+            }
+        }
+    }
+
+    // ---- Gravity half-kick: src/source_term/const_gravity.cpp:10-67 ---------------------------
+    void ConstGravity(std::vector<Real>& U, Real h) {
+        const Real g[3] = {gx, gy, gz};
+        for (int a = 0; a < 3; a++) {
+            if (g[a] == 0) continue;
+            const int m = MX + a;
+#pragma omp parallel for schedule(static)
+            for (int j = jb; j < je; j++) {
+                for (int i = ib; i < ie; i++) {
+                    U[at(ENG, j, i)] += U[at(m, j, i)] * g[a] * h;
+                    U[at(m, j, i)] += U[at(DEN, j, i)] * g[a] * h;
+                }
+            }
+        }
+    }
+
+    // ---- Integrators: src/evolution/{forward_euler,rk2,rk3}.cpp ------------------------------
+    void Stage(std::vector<Real>& in) {
+        UpdateBD(in);
+        ConsToPrim(in, prim);
+        CalFlux();
+    }
+    void Integrate() {
+        if (P.integrator == 1) {
+            Stage(cons); UpdateCons(cons, consTemp, 1.0, 1.0, 0, false);
+            cons.swap(consTemp);
+        } else if (P.integrator == 3) {
+            Stage(cons); UpdateCons(cons, consTemp, 1.0, 1.0, 0, false);
+            Stage(consTemp); UpdateCons(cons, consTemp, 0.75, 0.25, 0.25, true);
+            Stage(consTemp);
+            Real frac13 = Real(1.0) / Real(3.0);
+            UpdateCons(cons, consTemp, frac13, 2 * frac13, 2 * frac13, true);
+            cons.swap(consTemp);
+        } else {
+            Stage(cons); UpdateCons(cons, consTemp, 1.0, 1.0, 0, false);
+            Stage(consTemp); UpdateCons(cons, consTemp, 0.5, 0.5, 0.5, true);
+            cons.swap(consTemp);
+        }
+    }
+
+    // ---- Sim::Advance: src/sim.cpp:69-117 ---------------------------------------------------
+    int Advance(Real dtoutput) {
+        Real tNext = t + dtoutput;
+        dtUntilOutput = dtoutput;
+        int taken = 0;
+        log_t.clear(); log_dt.clear();
+        while (std::abs(tNext - t) > (Real)1e-12 && isContinue) {
+            cmax = (Real)1e-16;
+            CalCmax(cons, cmax);
+            dt = cfl * drmin / cmax;
+            dt = smin(dt, dtUntilOutput);
+            if (src_enrolled) ConstGravity(cons, Real(0.5) * dt);
+            Integrate();
+            if (src_enrolled) ConstGravity(cons, Real(0.5) * dt);
+            t += dt;
+            step++;
+            taken++;
+            dtUntilOutput = tNext - t;
+            if (P.stepmax > 0 && step >= P.stepmax) isContinue = false;
+            log_t.push_back((double)t); log_dt.push_back((double)dt);
+        }
+        return taken;
+    }
+};
+
+struct Holder {
+    int prec;
+    Oracle<double>* d = nullptr;
+    Oracle<float>* f = nullptr;
+};
+
+}  // namespace
+
+struct gko_sim { Holder h; };
+
+extern "C" {
+
+gko_sim* gko_create(const gko_params* p, int precision) {
+    try {
+        gko_sim* s = new gko_sim();
+        s->h.prec = precision;
+        if (precision == 32) s->h.f = new Oracle<float>(*p);
+        else s->h.d = new Oracle<double>(*p);
+        return s;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void gko_destroy(gko_sim* s) {
+    if (!s) return;
+    delete s->h.d; delete s->h.f; delete s;
+}
+
+void gko_set_cons(gko_sim* s, const void* c) {
+    if (s->h.f) std::memcpy(s->h.f->cons.data(), c, s->h.f->cons.size() * sizeof(float));
+    else std::memcpy(s->h.d->cons.data(), c, s->h.d->cons.size() * sizeof(double));
+}
+void gko_get_cons(const gko_sim* s, void* c) {
+    if (s->h.f) std::memcpy(c, s->h.f->cons.data(), s->h.f->cons.size() * sizeof(float));
+    else std::memcpy(c, s->h.d->cons.data(), s->h.d->cons.size() * sizeof(double));
+}
+int gko_advance(gko_sim* s, double dtoutput) {
+    return s->h.f ? s->h.f->Advance((float)dtoutput) : s->h.d->Advance(dtoutput);
+}
+double gko_time(const gko_sim* s) { return s->h.f ? (double)s->h.f->t : s->h.d->t; }
+int gko_step(const gko_sim* s) { return s->h.f ? s->h.f->step : s->h.d->step; }
+int gko_is_continue(const gko_sim* s) { return s->h.f ? s->h.f->isContinue : s->h.d->isContinue; }
+double gko_last_dt(const gko_sim* s) { return s->h.f ? (double)s->h.f->dt : s->h.d->dt; }
+void gko_update_bd(gko_sim* s) {
+    if (s->h.f) s->h.f->UpdateBD(s->h.f->cons); else s->h.d->UpdateBD(s->h.d->cons);
+}
+int gko_dt_log(const gko_sim* s, double* t_out, double* dt_out, int cap) {
+    const std::vector<double>& lt = s->h.f ? s->h.f->log_t : s->h.d->log_t;
+    const std::vector<double>& ld = s->h.f ? s->h.f->log_dt : s->h.d->log_dt;
+    int n = (int)std::min<size_t>(lt.size(), (size_t)cap);
+    for (int k = 0; k < n; k++) { t_out[k] = lt[k]; dt_out[k] = ld[k]; }
+    return (int)lt.size();
+}
+
+// ---- per-function entry points ----------------------------------------------------------------
+static gko_params unit_params(double gamma) {
+    gko_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.nx = 1; p.ny = 1; p.ng = 2;
+    p.xmax = p.ymax = p.zmax = 1.0;
+    p.gamma = gamma; p.cfl = 0.4; p.integrator = 2; p.rc_order = 2;
+    return p;
+}
+
+void gko_hllc(int n, const double* ul, const double* ur, int dir, double gamma, double* flux) {
+    Oracle<double> o(unit_params(gamma));
+    // rotation of src/flux/hllc.cpp:25-27
+    int rot[NV] = {DEN, 0, 0, 0, PRE};
+    for (int k = 0; k < 3; k++) rot[1 + k] = VX + (dir - VX + k) % 3;
+    for (int i = 0; i < n; i++) {
+        double wl[NV], wr[NV], f[NV];
+        for (int q = 0; q < NV; q++) { wl[q] = ul[rot[q] * n + i]; wr[q] = ur[rot[q] * n + i]; }
+        o.Hllc(wl, wr, f);
+        for (int q = 0; q < NV; q++) flux[rot[q] * n + i] = f[q];
+    }
+}
+
+void gko_plm_row(int m, const double* prim, double* ul, double* ur) {
+    for (int n = 0; n < NV; n++) {
+        for (int i = 0; i < m - 2; i++) {
+            double u1 = prim[n * m + i], u2 = prim[n * m + i + 1], u3 = prim[n * m + i + 2];
+            double s = Oracle<double>::VanLeer(u2 - u1, u3 - u2);
+            ul[n * (m + 1) + i + 2] = u2 + 0.5 * s;
+            ur[n * (m + 1) + i + 1] = u2 - 0.5 * s;
+        }
+    }
+}
+
+void gko_cons_to_prim(int n, const double* cons, double gamma, double* prim) {
+    gko_params p = unit_params(gamma);
+    Oracle<double> o(p);
+    // run ConsToPrim on a 1-row view: reuse the state arrays with lenx = n
+    o.lenx = n; o.leny = 1;
+    o.cons.assign(cons, cons + (size_t)NV * n);
+    o.prim.assign((size_t)NV * n, 0.0);
+    o.ConsToPrim(o.cons, o.prim);
+    std::memcpy(prim, o.prim.data(), sizeof(double) * NV * n);
+}
+
+void gko_prim_to_cons(int n, const double* prim, double gamma, double* cons) {
+    // src/eos/adiabatic.cpp:162-198
+    double gm1Rec = 1.0 / (gamma - 1.0);
+    for (int i = 0; i < n; i++) {
+        double d = prim[i], v1 = prim[n + i], v2 = prim[2 * n + i], v3 = prim[3 * n + i], p = prim[4 * n + i];
+        cons[i] = d;
+        cons[n + i] = d * v1;
+        cons[2 * n + i] = d * v2;
+        cons[3 * n + i] = d * v3;
+        cons[4 * n + i] = p * gm1Rec + 0.5 * d * (v1 * v1 + v2 * v2 + v3 * v3);
+    }
+}
+
+double gko_cmax(int n, const double* cons, double gamma, double cmax0) {
+    gko_params p = unit_params(gamma);
+    Oracle<double> o(p);
+    o.lenx = n; o.leny = 1; o.ib = 0; o.ie = n; o.jb = 0; o.je = 1;
+    o.cons.assign(cons, cons + (size_t)NV * n);
+    double c = cmax0;
+    o.CalCmax(o.cons, c);
+    return c;
+}
+
+}  // extern "C"

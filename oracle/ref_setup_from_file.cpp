@@ -1,0 +1,46 @@
+// ref_setup_from_file.cpp -- a problem setup in the reference's plug-in format
+// (void Sim::Setup(), src/sim.hpp:38; one setup file per binary, Makefile:82).
+//
+// TEST INFRASTRUCTURE ONLY: compiled with the reference sources into oracle/_ref/ to
+// produce golden fixtures.  It loads the active-region conserved state from the binary
+// file named by $GK_IC_FILE, so every implementation starts from bit-identical initial
+// conditions (the reference setups derive theirs from a truncated pi, src/gaukuk.hpp:19):
+//   int32 nvar(=5), int32 nx, int32 ny, float64 cons[5][ny][nx]
+// Optional config keys gx, gy, gz enroll constant gravity
+// (SourceTerm::EnrollConstGravityVector, src/source_term/source.cpp:23-31).
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gaukuk.hpp"
+#include "sim.hpp"
+
+namespace Gaukuk {
+
+void Sim::Setup() {
+    const char* path = std::getenv("GK_IC_FILE");
+    if (!path) throw std::runtime_error("GK_IC_FILE is not set");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    int hdr[3];
+    if (std::fread(hdr, sizeof(int), 3, f) != 3) throw std::runtime_error("short IC header");
+    if (hdr[0] != NVar || hdr[1] != grid.nx || hdr[2] != grid.ny)
+        throw std::runtime_error("IC file does not match NX/NY");
+    const size_t n = (size_t)NVar * grid.nx * grid.ny;
+    std::vector<double> buf(n);
+    if (std::fread(buf.data(), sizeof(double), n, f) != n) throw std::runtime_error("short IC data");
+    std::fclose(f);
+    const int k = grid.kb;
+    for (int v = 0; v < NVar; v++)
+        for (int j = 0; j < grid.ny; j++)
+            for (int i = 0; i < grid.nx; i++)
+                cons(v, k, grid.jb + j, grid.ib + i) = (Real)buf[((size_t)v * grid.ny + j) * grid.nx + i];
+    Real gx = Config::getInstance().get("gx", 0.0);
+    Real gy = Config::getInstance().get("gy", 0.0);
+    Real gz = Config::getInstance().get("gz", 0.0);
+    srcTerm.EnrollConstGravityVector(gx, gy, gz);
+}
+
+}  // namespace Gaukuk

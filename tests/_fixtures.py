@@ -1,0 +1,31 @@
+"""Loading helpers for the golden fixtures in tests/golden/ (made by make_golden.py)."""
+import glob
+import json
+import os
+
+import numpy as np
+
+from gaukuk_amd.problems import Problem
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def run_cases():
+    return sorted(os.path.basename(p)[4:-4] for p in glob.glob(os.path.join(GOLDEN, "run_*.npz")))
+
+
+def load_run(name):
+    z = np.load(os.path.join(GOLDEN, f"run_{name}.npz"), allow_pickle=False)
+    m = json.loads(str(z["meta"]))
+    prob = Problem(
+        name=name, nx=int(m["NX"]), ny=int(m["NY"]),
+        extent=(m["xmin"], m["xmax"], m["ymin"], m["ymax"], m["zmin"], m["zmax"]),
+        gamma=m["gamma"], cfl=m["CFL"], integrator=int(m["integrator"]), rc_order=int(m["RcOrder"]),
+        stepmax=int(m["stepmax"]), tmax=m["tmax"], dtoutput=m["dtoutput"],
+        bc=(int(m["xleft"]), int(m["xright"]), int(m["yleft"]), int(m["yright"])),
+        g=(m["gx"], m["gy"], m["gz"]), ng=int(m["NGhost"]), cons=z["ic"].copy())
+    return prob, z["final"].copy(), float(z["t"]), int(z["steps"])
+
+
+def load_vec(name):
+    return np.load(os.path.join(GOLDEN, f"vec_{name}.npz"), allow_pickle=False)

@@ -1,0 +1,142 @@
+"""Write golden fixtures from the compiled reference (oracle/_ref, built by `make -C oracle ref`).
+
+Run in a container that has the reference sources:  python tests/golden/make_golden.py
+It produces, next to this script:
+  run_<case>.npz   whole-run fixtures: initial conserved state (active cells), the .in values,
+                   the reference's final active state, final time and step count
+  vec_<op>.npz     per-function vectors from oracle/_ref/ref_vectors (reference objects)
+The reference binary reads its initial state through oracle/ref_setup_from_file.cpp, so the
+ICs are exactly the arrays stored in each fixture.
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+sys.path.insert(0, ROOT)
+
+from gaukuk_amd import problems as P  # noqa: E402
+from gaukuk_amd.config import write_in  # noqa: E402
+from gaukuk_amd.gkio import GaukukOutput  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+
+def cases():
+    out = {}
+    out["sod256"] = P.sod(256)
+    out["sod128_fe_rc1"] = P.sod(128, integrator=1, rc_order=1, stepmax=10)
+    out["riemann_rk3_refl_outflow"] = P.riemann_1d(160, (1.0, 0.75, 1.0), (0.125, 0.0, 0.1), x0=0.3,
+                                                   integrator=3, bc=(2, 3, 0, 0), stepmax=60)
+    out["vacuum_rk2"] = P.riemann_1d(128, (1.0, -4.0, 0.05), (1.0, 4.0, 0.05), integrator=2, stepmax=40)
+    out["kh64_rk3"] = P.kelvin_helmholtz(64, seed=7, stepmax=20)
+    out["kh48x40_rk2_rc1"] = P.kelvin_helmholtz(48, ny=40, seed=11, integrator=2, rc_order=1, stepmax=15)
+    out["kh32_fe"] = P.kelvin_helmholtz(32, seed=3, integrator=1, stepmax=10)
+    out["rt32x96_rk2_g"] = P.rayleigh_taylor(32, 96, stepmax=20)
+    out["blast64_bc1"] = P.blast(64, 64, p_ratio=50.0, radius=0.15, bc=1, stepmax=30)
+    out["blast64_bc3"] = P.blast(64, 64, p_ratio=80.0, radius=0.2, bc=3, stepmax=40)
+    mixed = P.blast(48, 56, p_ratio=30.0, radius=0.12, center=(0.3, 0.6), bc=(3, 0, 2, 3), stepmax=40)
+    # background flow into the domain through the outflow faces exercises the inflow clamp
+    # (src/boundary/outflow.cpp:48-58)
+    rho, mx, my, mz, e = mixed.cons
+    vx0, vy0 = 0.3, -0.2
+    mixed.cons = P.prim_to_cons(rho, np.full_like(rho, vx0), np.full_like(rho, vy0), 0 * rho,
+                                (e - 0.0) * (mixed.gamma - 1.0), mixed.gamma)
+    out["blast48x56_mixed"] = mixed.with_(name="blast_mixed")
+    out["rt24x64_rk3_gx"] = P.rayleigh_taylor(24, 64, stepmax=12).with_(g=(0.05, -0.1, 0.0), integrator=3,
+                                                                        bc=(1, 1, 3, 2))
+    return out
+
+
+def run_reference(prob, workdir):
+    ic = os.path.join(workdir, "ic.bin")
+    with open(ic, "wb") as f:
+        np.array([5, prob.nx, prob.ny], np.int32).tofile(f)
+        np.ascontiguousarray(prob.cons, np.float64).tofile(f)
+    inp = os.path.join(workdir, "case.in")
+    write_in(inp, prob.in_values())
+    env = dict(os.environ, GK_IC_FILE=ic, OMP_NUM_THREADS="4")
+    r = subprocess.run([os.path.join(REF, "gaukuk_ref"), "-i", inp], cwd=workdir, env=env,
+                       capture_output=True, text=True, check=True)
+    steps = int(re.search(r"total steps =\s*(\d+)", r.stdout).group(1))
+    outs = sorted(f for f in os.listdir(workdir) if f.startswith("cons_"))
+    last = GaukukOutput(os.path.join(workdir, outs[-1]))
+    return last.active().copy(), float(last.t), steps, len(outs) - 1
+
+
+def vectors(rng):
+    def ref_op(op, payload_ints, arrays, n_cfg, nout):
+        with tempfile.TemporaryDirectory() as d:
+            cfg = os.path.join(d, "v.in")
+            write_in(cfg, {"gamma": 1.4, "NX": n_cfg, "NY": 1, "NZ": 1, "NGhost": 2})
+            fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+            with open(fin, "wb") as f:
+                np.array(payload_ints, np.int32).tofile(f)
+                for a in arrays:
+                    np.ascontiguousarray(a, np.float64).tofile(f)
+            subprocess.run([os.path.join(REF, "ref_vectors"), cfg, op, fin, fout], check=True)
+            return np.fromfile(fout, np.float64, nout)
+
+    def states(n):
+        rho = np.exp(rng.uniform(np.log(0.05), np.log(20.0), n))
+        v = rng.uniform(-3, 3, (3, n))
+        p = np.exp(rng.uniform(np.log(0.01), np.log(20.0), n))
+        return np.vstack([rho, v, p])
+
+    n = 4096
+    ul, ur = states(n), states(n)
+    # special rows: identical states, strongly supersonic both ways, a stationary contact
+    ur[:, :64] = ul[:, :64]
+    ul[1, 64:128] = 8.0; ur[1, 64:128] = 8.0
+    ul[1, 128:192] = -8.0; ur[1, 128:192] = -8.0
+    ul[1:4, 192:256] = 0.0; ur[1:4, 192:256] = 0.0; ur[4, 192:256] = ul[4, 192:256]
+    res = {}
+    for d in (1, 2, 3):
+        f = ref_op("hllc", [n, d], [ul, ur], 8, 5 * n).reshape(5, n)
+        res[f"hllc_dir{d}"] = f
+    vec = {"hllc_ul": ul, "hllc_ur": ur, **res}
+    np.savez_compressed(os.path.join(HERE, "vec_hllc.npz"), **vec)
+
+    m = 2048
+    prim = states(m)
+    prim[:, 100:140] = prim[:, 100:101]          # flat stretch: zero slopes
+    prim[0, 300:340] = np.arange(40) % 2 + 1.0   # alternating: extrema, multi <= 0
+    out = ref_op("plm", [m, 0], [prim], 8, 2 * 5 * (m + 1))
+    np.savez_compressed(os.path.join(HERE, "vec_plm.npz"), prim=prim,
+                        ul=out[:5 * (m + 1)].reshape(5, m + 1), ur=out[5 * (m + 1):].reshape(5, m + 1))
+
+    cons = P.prim_to_cons(*states(n), 1.4)
+    cons[0, :32] = rng.uniform(-1e-15, 1e-17, 32)        # below the density floor
+    cons[4, 32:64] = 0.1 * cons[4, 32:64] - 5.0          # E < KE: pressure floor
+    c2p = ref_op("c2p", [n, 0], [cons], n, 5 * n).reshape(5, n)
+    p2c_in = states(n)
+    p2c = ref_op("p2c", [n, 0], [p2c_in], n, 5 * n).reshape(5, n)
+    cm = ref_op("cmax", [n, 0], [cons], n, 1)
+    np.savez_compressed(os.path.join(HERE, "vec_eos.npz"), cons=cons, c2p=c2p, p2c_in=p2c_in, p2c=p2c,
+                        cmax=cm)
+
+
+def main():
+    if not os.path.exists(os.path.join(REF, "gaukuk_ref")):
+        sys.exit("oracle/_ref/gaukuk_ref missing: run `make -C oracle ref` where the reference exists")
+    for name, prob in cases().items():
+        with tempfile.TemporaryDirectory() as d:
+            final, t, steps, nout = run_reference(prob, d)
+        meta = {k: v for k, v in prob.in_values().items()}
+        np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), ic=prob.cons, final=final,
+                            t=np.array(t), steps=np.array(steps), meta=json.dumps(meta))
+        print(f"{name:28s} {prob.nx}x{prob.ny} steps={steps} t={t!r}")
+    vectors(np.random.default_rng(20251203))
+    print("vectors written")
+
+
+if __name__ == "__main__":
+    main()
