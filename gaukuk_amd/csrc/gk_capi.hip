@@ -1,0 +1,539 @@
+// gk_capi.hip -- host runtime behind include/gaukuk.h.
+//
+// A context owns three state buffers (SoA planes of the active region), the device clock, a
+// step log and reduction scratch.  Integrators are tables of stage launches selected once at
+// creation, as the reference selects its pointer-to-member integrator (src/sim.cpp:60-66).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gaukuk.h"
+#include "gk_kernels.hip"
+
+using namespace gk;
+
+#define GK_STR2(x) #x
+#define GK_STR(x) GK_STR2(x)
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct StageSpec {
+    int in, base, out;  // buffer indices 0 (A = U^n), 1 (B), 2 (C)
+    bool three;
+    double c1, c2, c3;
+    bool first, last;
+};
+
+}  // namespace
+
+struct gk_ctx {
+    gk_grid grid;
+    gk_phys phys;
+    int prec = 64;
+    size_t esz = 8;
+    int nx = 0, ny = 0, pitch = 0;
+    long long plane = 0;
+    void* buf[3] = {nullptr, nullptr, nullptr};
+    DevClock* clock = nullptr;
+    StepRecord* log = nullptr;
+    int log_cap = 4096;
+    double* partial = nullptr;
+    double* sums = nullptr;
+    int sum_blocks = 512;
+    DevClock* hclock = nullptr;  // pinned mirror
+    hipStream_t stream = nullptr;
+    int seg_rows = 0, strips = 0, segs = 0, nwaves = 0, nblocks = 0;
+    bool uploaded = false;
+    bool gravity = false;
+    double dxRec = 0, dyRec = 0, drmin = 0;
+    int bc[4] = {0, 0, 0, 0};
+    std::vector<StageSpec> stages;
+    std::string err;
+};
+
+#define GK_TRY_HIP(ctx, call)                                                              \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
+            return GK_EHIP;                                                                \
+        }                                                                                  \
+    } while (0)
+
+static gk_status fail(gk_ctx* c, gk_status s, const std::string& m) {
+    c->err = m;
+    return s;
+}
+
+static void plan_waves(gk_ctx* c, int seg_rows) {
+    c->strips = (c->nx + kStrip - 1) / kStrip;
+    if (seg_rows <= 0) {
+        const int target = 4096;  // ~16 waves per CU on 256 CUs
+        int segs = (target + c->strips - 1) / c->strips;
+        seg_rows = (c->ny + segs - 1) / segs;
+        if (seg_rows < 8) seg_rows = 8;
+    }
+    if (seg_rows > c->ny) seg_rows = c->ny;
+    c->seg_rows = seg_rows;
+    c->segs = (c->ny + seg_rows - 1) / seg_rows;
+    c->nwaves = c->strips * c->segs;
+    int nb = (c->nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
+    c->nblocks = (nb + 7) / 8 * 8;  // a multiple of 8 keeps the XCD remap a bijection
+}
+
+static void build_integrator(gk_ctx* c) {
+    c->stages.clear();
+    const int it = c->phys.integrator;
+    if (it == 1) {  // ForwardEuler_: src/evolution/forward_euler.cpp:11-17
+        c->stages.push_back({0, 0, 1, false, 1.0, 1.0, 0.0, true, true});
+    } else if (it == 3) {  // RK3_: src/evolution/rk3.cpp:11-28
+        const double f13 = 1.0 / 3.0;
+        c->stages.push_back({0, 0, 1, false, 1.0, 1.0, 0.0, true, false});
+        c->stages.push_back({1, 0, 2, true, 0.75, 0.25, 0.25, false, false});
+        c->stages.push_back({2, 0, 0, true, f13, 2 * f13, 2 * f13, false, true});
+    } else {  // RK2_ (also the reference's default branch): src/evolution/rk2.cpp:11-22
+        c->stages.push_back({0, 0, 1, false, 1.0, 1.0, 0.0, true, false});
+        c->stages.push_back({1, 0, 0, true, 0.5, 0.5, 0.5, false, true});
+    }
+}
+
+template <class R>
+static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
+    StageArgs<R> a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = (const R*)c->buf[s.in];
+    a.base = (const R*)c->buf[s.base];
+    a.out = (R*)c->buf[s.out];
+    a.plane = c->plane;
+    a.pitch = c->pitch; a.nx = c->nx; a.ny = c->ny;
+    a.seg_rows = c->seg_rows; a.strips = c->strips; a.nwaves = c->nwaves;
+    for (int k = 0; k < 4; k++) a.bc[k] = c->bc[k];
+    // RK3's last coefficients are frac13 = 1.0/3.0 and 2*frac13 (src/evolution/rk3.cpp:25-26);
+    // rounding the double values to float equals evaluating them in float.
+    a.c1 = R(s.c1); a.c2 = R(s.c2); a.c3 = R(s.c3);
+    a.kick_in = c->gravity && s.first;
+    a.kick_base = c->gravity;
+    a.pingpong = c->phys.integrator == 1;
+    a.kick_out = c->gravity && s.last;
+    a.want_cmax = s.last;
+    a.gx = R(c->phys.g[0]); a.gy = R(c->phys.g[1]); a.gz = R(c->phys.g[2]);
+    a.dxRec = c->dxRec; a.dyRec = c->dyRec;
+    a.eos = EosPolicy<R>::make(c->phys.gamma);
+    a.clock = c->clock;
+    return a;
+}
+
+template <class R>
+static void launch_stage(gk_ctx* c, const StageSpec& s) {
+    StageArgs<R> a = make_args<R>(c, s);
+    dim3 grid(c->nblocks), block(kBlock);
+    const bool plm = c->phys.rc_order != 1;
+    if (s.three) {
+        if (plm) hipLaunchKernelGGL((stage_kernel<R, VanLeer, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, FirstOrder, true>), grid, block, 0, c->stream, a);
+    } else {
+        if (plm) hipLaunchKernelGGL((stage_kernel<R, VanLeer, false>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, FirstOrder, false>), grid, block, 0, c->stream, a);
+    }
+}
+
+static void launch_stage_any(gk_ctx* c, const StageSpec& s) {
+    if (c->prec == 32) launch_stage<float>(c, s);
+    else launch_stage<double>(c, s);
+}
+
+static void enqueue_step(gk_ctx* c) {
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
+    for (const StageSpec& s : c->stages) launch_stage_any(c, s);
+    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap,
+                       c->phys.integrator == 1 ? 1 : 0);
+}
+
+static gk_status read_clock(gk_ctx* c) {
+    GK_TRY_HIP(c, hipMemcpyAsync(c->hclock, c->clock, sizeof(DevClock), hipMemcpyDeviceToHost, c->stream));
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    return GK_OK;
+}
+
+static gk_status write_clock(gk_ctx* c) {
+    GK_TRY_HIP(c, hipMemcpyAsync(c->clock, c->hclock, sizeof(DevClock), hipMemcpyHostToDevice, c->stream));
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    return GK_OK;
+}
+
+// Recompute the max wave speed of the current state into the slot the next step reads.
+static gk_status refresh_cmax(gk_ctx* c) {
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    unsigned long long one = 0;
+    double floor_c = 1e-16;
+    std::memcpy(&one, &floor_c, sizeof one);
+    c->hclock->cmax_bits[c->hclock->parity] = one;
+    if ((s = write_clock(c)) != GK_OK) return s;
+    const int cur = c->hclock->cur;
+    const void* u = c->buf[cur ? 1 : 0];
+    dim3 grid(1024), block(kBlock);
+    if (c->prec == 32)
+        hipLaunchKernelGGL(cmax_kernel<float>, grid, block, 0, c->stream, (const float*)u, c->plane, c->pitch, c->nx,
+                           c->ny, EosPolicy<float>::make(c->phys.gamma), c->clock);
+    else
+        hipLaunchKernelGGL(cmax_kernel<double>, grid, block, 0, c->stream, (const double*)u, c->plane, c->pitch, c->nx,
+                           c->ny, EosPolicy<double>::make(c->phys.gamma), c->clock);
+    GK_TRY_HIP(c, hipGetLastError());
+    return GK_OK;
+}
+
+static int current_buffer(gk_ctx* c) { return c->hclock->cur ? 1 : 0; }
+
+// Reference layout helpers: Real[5][lenz][leny][lenx], active plane k = nghost.
+template <class R>
+static void ref_to_active(const gk_ctx* c, const R* ref, double* act) {
+    const int g = c->grid.nghost, lx = c->nx + 2 * g, ly = c->ny + 2 * g, lz = 1 + 2 * g;
+    for (int n = 0; n < 5; n++)
+        for (int j = 0; j < c->ny; j++)
+            for (int i = 0; i < c->nx; i++)
+                act[((size_t)n * c->ny + j) * c->nx + i] =
+                    (double)ref[(((size_t)n * lz + g) * ly + (j + g)) * lx + (i + g)];
+}
+
+template <class R>
+static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = 1 + 2 * g;
+    std::memset(ref, 0, sizeof(R) * 5 * (size_t)lx * ly * lz);
+    auto at = [&](int n, int j, int i) -> R& { return ref[(((size_t)n * lz + g) * ly + j) * lx + i]; };
+    for (int n = 0; n < 5; n++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) at(n, j + g, i + g) = (R)act[((size_t)n * ny + j) * nx + i];
+    // ghost fill over the active tangential range, x faces then y faces (src/boundary/boundary.cpp:127-142)
+    for (int j = g; j < g + ny; j++)
+        for (int i = 0; i < lx; i++) {
+            if (i >= g && i < g + nx) continue;
+            int kind;
+            const int si = bc_remap(i - g, nx, c->bc[0], c->bc[1], kind) + g;
+            R u[5];
+            for (int n = 0; n < 5; n++) u[n] = at(n, j, si);
+            bc_transform(u, 1, kind);
+            for (int n = 0; n < 5; n++) at(n, j, i) = u[n];
+        }
+    if (ny > 1)
+        for (int j = 0; j < ly; j++) {
+            if (j >= g && j < g + ny) continue;
+            for (int i = g; i < g + nx; i++) {
+                int kind;
+                const int sj = bc_remap(j - g, ny, c->bc[2], c->bc[3], kind) + g;
+                R u[5];
+                for (int n = 0; n < 5; n++) u[n] = at(n, sj, i);
+                bc_transform(u, 2, kind);
+                for (int n = 0; n < 5; n++) at(n, j, i) = u[n];
+            }
+        }
+}
+
+
+extern "C" {
+
+gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
+    if (!grid || !phys || !out) { g_create_error = "null argument"; return GK_EINVAL; }
+    *out = nullptr;
+    // src/grid.hpp:23-25: non-positive sizes are a configuration error
+    if (grid->nx <= 0 || grid->ny <= 0 || grid->nz <= 0 || grid->nghost < 0) {
+        g_create_error = "Invalid grid parameters from config"; return GK_EINVAL;
+    }
+    if (grid->nz != 1) { g_create_error = "only 2D/1D grids (NZ = 1) run on this path"; return GK_EINVAL; }
+    if (grid->nghost < 2) { g_create_error = "NGhost must be >= 2 for the PLM stencil"; return GK_EINVAL; }
+    if (grid->nx < 2) { g_create_error = "NX must be >= 2"; return GK_EINVAL; }
+    if (phys->precision != 64 && phys->precision != 32) { g_create_error = "precision must be 64 or 32"; return GK_EINVAL; }
+    for (int f = 0; f < 4; f++)
+        if (phys->bc[f] == 4) { g_create_error = "fixed boundaries (type 4) are not supported"; return GK_EINVAL; }
+    if (!(phys->gamma > 1.0)) { g_create_error = "gamma must exceed 1"; return GK_EINVAL; }
+    if ((long long)grid->nx * grid->ny * 5 >= (1LL << 40)) { g_create_error = "grid too large"; return GK_EINVAL; }
+
+    gk_ctx* c = new gk_ctx();
+    c->grid = *grid;
+    c->phys = *phys;
+    c->prec = phys->precision;
+    c->esz = c->prec == 32 ? 4 : 8;
+    c->nx = grid->nx; c->ny = grid->ny; c->pitch = grid->nx;
+    c->plane = (long long)c->pitch * c->ny;
+    for (int f = 0; f < 4; f++) c->bc[f] = (phys->bc[f] >= 0 && phys->bc[f] <= 3) ? phys->bc[f] : 0;
+    if (c->ny == 1) { c->bc[2] = 0; c->bc[3] = 0; }  // y fluxes are multiplied by dyRec = 0
+    c->gravity = phys->g[0] != 0 || phys->g[1] != 0 || phys->g[2] != 0;
+    // Domain (src/sim.cpp:16-36)
+    const double dx = (grid->xmax - grid->xmin) / grid->nx;
+    const double dy = (grid->ymax - grid->ymin) / grid->ny;
+    const double dz = (grid->zmax - grid->zmin) / grid->nz;
+    c->dxRec = 1. / dx;
+    c->dyRec = grid->ny <= 1 ? 0.0 : 1. / dy;
+    c->drmin = smin(smin(dx, dy), dz);
+    plan_waves(c, 0);
+    build_integrator(c);
+
+    auto bail = [&](gk_status s, const std::string& m) {
+        g_create_error = m;
+        gk_destroy(c);
+        return s;
+    };
+    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(GK_EHIP, "hipStreamCreate failed");
+    const size_t bytes = (size_t)c->plane * 5 * c->esz;
+    const int nbuf = phys->integrator == 3 ? 3 : 2;
+    for (int k = 0; k < nbuf; k++)
+        if (hipMalloc(&c->buf[k], bytes) != hipSuccess) return bail(GK_ENOMEM, "state allocation failed");
+    if (nbuf == 2) c->buf[2] = c->buf[1];
+    if (hipMalloc(&c->clock, sizeof(DevClock)) != hipSuccess ||
+        hipMalloc(&c->log, sizeof(StepRecord) * c->log_cap) != hipSuccess ||
+        hipMalloc(&c->partial, sizeof(double) * 5 * c->sum_blocks) != hipSuccess ||
+        hipMalloc(&c->sums, sizeof(double) * 5) != hipSuccess ||
+        hipHostMalloc((void**)&c->hclock, sizeof(DevClock)) != hipSuccess)
+        return bail(GK_ENOMEM, "auxiliary allocation failed");
+    // Sim::Sim initial scalars (src/sim.cpp:45-47)
+    DevClock h;
+    std::memset(&h, 0, sizeof h);
+    h.t = 0; h.dt = 1e10; h.t_next = 0; h.dt_until_output = 1e10;
+    double floor_c = 1e-16;
+    std::memcpy(&h.cmax_bits[0], &floor_c, 8);
+    std::memcpy(&h.cmax_bits[1], &floor_c, 8);
+    h.step = 0; h.stepmax = phys->stepmax; h.is_continue = 1; h.active = 0; h.parity = 0; h.cur = 0;
+    h.cfl = phys->cfl; h.drmin = c->drmin;
+    *c->hclock = h;
+    if (hipMemcpy(c->clock, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) return bail(GK_EHIP, "clock init failed");
+    *out = c;
+    return GK_OK;
+}
+
+void gk_destroy(gk_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* seen[3] = {nullptr, nullptr, nullptr};
+    for (int k = 0; k < 3; k++) {
+        bool dup = false;
+        for (int q = 0; q < k; q++) dup |= seen[q] == c->buf[k];
+        if (c->buf[k] && !dup) (void)hipFree(c->buf[k]);
+        seen[k] = c->buf[k];
+    }
+    if (c->clock) (void)hipFree(c->clock);
+    if (c->log) (void)hipFree(c->log);
+    if (c->partial) (void)hipFree(c->partial);
+    if (c->sums) (void)hipFree(c->sums);
+    if (c->hclock) (void)hipHostFree(c->hclock);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+gk_status gk_upload_active(gk_ctx* c, const double* host) {
+    if (!c || !host) return GK_EINVAL;
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    c->hclock->cur = 0;
+    if ((s = write_clock(c)) != GK_OK) return s;
+    const size_t n = (size_t)c->plane * 5;
+    if (c->prec == 32) {
+        std::vector<float> tmp(n);
+        for (size_t k = 0; k < n; k++) tmp[k] = (float)host[k];
+        GK_TRY_HIP(c, hipMemcpy(c->buf[0], tmp.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+        GK_TRY_HIP(c, hipMemcpy(c->buf[0], host, n * 8, hipMemcpyHostToDevice));
+    }
+    c->uploaded = true;
+    return refresh_cmax(c);
+}
+
+gk_status gk_download_active(gk_ctx* c, double* host) {
+    if (!c || !host) return GK_EINVAL;
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    const size_t n = (size_t)c->plane * 5;
+    const void* src = c->buf[current_buffer(c)];
+    if (c->prec == 32) {
+        std::vector<float> tmp(n);
+        GK_TRY_HIP(c, hipMemcpy(tmp.data(), src, n * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; k++) host[k] = tmp[k];
+    } else {
+        GK_TRY_HIP(c, hipMemcpy(host, src, n * 8, hipMemcpyDeviceToHost));
+    }
+    return GK_OK;
+}
+
+gk_status gk_upload_cons(gk_ctx* c, const void* host) {
+    if (!c || !host) return GK_EINVAL;
+    std::vector<double> act((size_t)c->plane * 5);
+    if (c->prec == 32) ref_to_active(c, (const float*)host, act.data());
+    else ref_to_active(c, (const double*)host, act.data());
+    return gk_upload_active(c, act.data());
+}
+
+gk_status gk_download_cons(gk_ctx* c, void* host) {
+    if (!c || !host) return GK_EINVAL;
+    std::vector<double> act((size_t)c->plane * 5);
+    gk_status s = gk_download_active(c, act.data());
+    if (s != GK_OK) return s;
+    if (c->prec == 32) active_to_ref(c, act.data(), (float*)host);
+    else active_to_ref(c, act.data(), (double*)host);
+    return GK_OK;
+}
+
+gk_status gk_advance(gk_ctx* c, double dtoutput, double* t, int* step) {
+    if (!c) return GK_EINVAL;
+    if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, dtoutput);
+    GK_TRY_HIP(c, hipGetLastError());
+    int chunk = 4;
+    for (;;) {
+        for (int k = 0; k < chunk; k++) enqueue_step(c);
+        GK_TRY_HIP(c, hipGetLastError());
+        gk_status s = read_clock(c);
+        if (s != GK_OK) return s;
+        if (!c->hclock->active) break;
+        if (chunk < 256) chunk *= 2;
+    }
+    if (t) *t = c->hclock->t;
+    if (step) *step = c->hclock->step;
+    return GK_OK;
+}
+
+gk_status gk_get_time(gk_ctx* c, double* t, double* dt, int* step, int* is_continue) {
+    if (!c) return GK_EINVAL;
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    if (t) *t = c->hclock->t;
+    if (dt) *dt = c->hclock->dt;
+    if (step) *step = c->hclock->step;
+    if (is_continue) *is_continue = c->hclock->is_continue;
+    return GK_OK;
+}
+
+gk_status gk_step_log(gk_ctx* c, int* steps, double* t, double* dt, int cap, int* count) {
+    if (!c || cap < 0) return GK_EINVAL;
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    std::vector<StepRecord> all(c->log_cap);
+    GK_TRY_HIP(c, hipMemcpy(all.data(), c->log, sizeof(StepRecord) * c->log_cap, hipMemcpyDeviceToHost));
+    const int total = c->hclock->step;
+    int n = total < c->log_cap ? total : c->log_cap;
+    if (n > cap) n = cap;
+    for (int k = 0; k < n; k++) {
+        const StepRecord& r = all[(total - n + k) % c->log_cap];
+        if (steps) steps[k] = r.step;
+        if (t) t[k] = r.t;
+        if (dt) dt[k] = r.dt;
+    }
+    if (count) *count = n;
+    return GK_OK;
+}
+
+gk_status gk_run_steps(gk_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return GK_EINVAL;
+    if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
+    for (int k = 0; k < nsteps; k++) enqueue_step(c);
+    GK_TRY_HIP(c, hipGetLastError());
+    return GK_OK;
+}
+
+gk_status gk_synchronize(gk_ctx* c) {
+    if (!c) return GK_EINVAL;
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    return GK_OK;
+}
+
+gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kernel_ms[3], int launches[3]) {
+    if (!c || nsteps < 0) return GK_EINVAL;
+    if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    // one event before and after every launch of the step: kind 2 = clock kernels
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    ev.reserve((size_t)nsteps * 10 + 2);
+    auto rec = [&]() {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, c->stream);
+        ev.push_back(e);
+    };
+    rec();
+    for (int k = 0; k < nsteps; k++) {
+        hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
+        rec(); kind.push_back(2);
+        for (const StageSpec& s : c->stages) {
+            launch_stage_any(c, s);
+            rec(); kind.push_back(s.three ? 1 : 0);
+        }
+        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap,
+                           c->phys.integrator == 1 ? 1 : 0);
+        rec(); kind.push_back(2);
+    }
+    GK_TRY_HIP(c, hipGetLastError());
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    double kms[3] = {0, 0, 0};
+    int cnt[3] = {0, 0, 0};
+    for (size_t q = 0; q + 1 < ev.size(); q++) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
+        kms[kind[q]] += ms;
+        cnt[kind[q]] += 1;
+    }
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (total_ms) *total_ms = tot;
+    for (int k = 0; k < 3; k++) {
+        if (kernel_ms) kernel_ms[k] = kms[k];
+        if (launches) launches[k] = cnt[k];
+    }
+    return GK_OK;
+}
+
+gk_status gk_cons_sums(gk_ctx* c, double out[5]) {
+    if (!c || !out) return GK_EINVAL;
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    const void* u = c->buf[current_buffer(c)];
+    if (c->prec == 32)
+        hipLaunchKernelGGL(sums_partial_kernel<float>, dim3(c->sum_blocks), dim3(256), 0, c->stream, (const float*)u,
+                           c->plane, c->pitch, c->nx, c->ny, c->partial);
+    else
+        hipLaunchKernelGGL(sums_partial_kernel<double>, dim3(c->sum_blocks), dim3(256), 0, c->stream, (const double*)u,
+                           c->plane, c->pitch, c->nx, c->ny, c->partial);
+    hipLaunchKernelGGL(sums_final_kernel, dim3(1), dim3(64), 0, c->stream, c->partial, c->sum_blocks, c->sums);
+    GK_TRY_HIP(c, hipGetLastError());
+    GK_TRY_HIP(c, hipMemcpyAsync(out, c->sums, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    return GK_OK;
+}
+
+gk_status gk_check_finite(gk_ctx* c) {
+    if (!c) return GK_EINVAL;
+    std::vector<double> act((size_t)c->plane * 5);
+    gk_status s = gk_download_active(c, act.data());
+    if (s != GK_OK) return s;
+    for (double v : act)
+        if (!std::isfinite(v)) return fail(c, GK_ENUMERIC, "non-finite conserved value");
+    return GK_OK;
+}
+
+gk_status gk_set_tile(gk_ctx* c, int seg_rows) {
+    if (!c || seg_rows < 0) return GK_EINVAL;
+    plan_waves(c, seg_rows);
+    return GK_OK;
+}
+
+gk_status gk_get_tile(gk_ctx* c, int* seg_rows, int* strips, int* waves) {
+    if (!c) return GK_EINVAL;
+    if (seg_rows) *seg_rows = c->seg_rows;
+    if (strips) *strips = c->strips;
+    if (waves) *waves = c->nwaves;
+    return GK_OK;
+}
+
+const char* gk_last_error(const gk_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+const char* gk_version(void) {
+    return "gaukuk_amd 0.1 gfx950 eos=" GK_STR(GK_EOS) " flux=" GK_STR(GK_FLUX);
+}
+
+}  // extern "C"

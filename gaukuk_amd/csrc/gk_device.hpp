@@ -1,0 +1,68 @@
+// gk_device.hpp -- device-side data shared by the kernels and the host runtime.
+#pragma once
+
+#include <stdint.h>
+
+#include "gk_physics.hpp"
+
+// Compile-time policy choice (configure --eos/--flux).  The reference links one EOS and one flux
+// source file per build (Makefile:64-90); here they are template arguments.
+#ifndef GK_EOS
+#define GK_EOS Adiabatic
+#endif
+#ifndef GK_FLUX
+#define GK_FLUX Hllc
+#endif
+
+namespace gk {
+
+template <class R> using EosPolicy = GK_EOS<R>;
+using RiemannPolicy = GK_FLUX;
+
+// Device-resident time state, so that whole steps (dt included) run without a host round
+// trip and can be captured in a HIP graph.  Mirrors the scalars of Sim::Advance
+// (src/sim.cpp:69-117): t, dt, dtUntilOutput, tNext, step, stepMax, isContinue.
+struct DevClock {
+    double t, dt, t_next, dt_until_output;
+    unsigned long long cmax_bits[2];  // running max of |v|+c_s as IEEE bits, indexed by parity
+    int step, stepmax, is_continue, active;
+    int parity;      // which cmax slot the current step reads
+    int cur;         // forward Euler: 1 when the current state lives in the second buffer
+    double cfl, drmin;
+};
+
+struct StepRecord {
+    double t, dt;
+    int step, pad;
+};
+
+// Arguments of one fused stage launch.  All arrays are SoA planes of the active region only:
+// element (n, j, i) at n*plane + j*pitch + i.
+template <class R>
+struct StageArgs {
+    const R* in;        // stage input U_s: fluxes are computed from it (halo via BC remap)
+    const R* base;      // U^n for the three-coefficient combine (unused for two coefficients)
+    R* out;             // stage output (may alias base, never in)
+    long long plane;
+    int pitch, nx, ny;
+    int seg_rows;       // rows per wave segment
+    int strips, nwaves; // wave grid: strips x segments
+    int bc[4];          // xleft, xright, yleft, yright (ny == 1: y forced to type 0)
+    R c1, c2, c3;       // combine coefficients
+    int kick_in;        // stage input is U^n: apply the first gravity half-kick on load
+    int kick_base;      // three-coefficient base is U^n: apply the first half-kick on load
+    int pingpong;       // forward Euler: swap in/out when clock->cur is set
+    int kick_out;       // last stage: apply the second half-kick to the result
+    int want_cmax;      // last stage: accumulate next step's max wave speed
+    R gx, gy, gz;
+    double dxRec, dyRec;
+    EosPolicy<R> eos;
+    DevClock* clock;
+};
+
+constexpr int kWave = 64;
+constexpr int kHalo = 2;
+constexpr int kStrip = kWave - 2 * kHalo;  // owned columns per wave
+constexpr int kBlock = 256;                // threads per workgroup (4 waves)
+
+}  // namespace gk

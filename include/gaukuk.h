@@ -1,0 +1,110 @@
+/*
+ * gaukuk.h -- C ABI of the MI355X hydro library (libgaukuk_hip.so).
+ *
+ * The reference has no FFI: its hot path is the C++ class Sim (src/sim.hpp:23-58) whose
+ * Advance() (src/sim.cpp:69-117) runs the integrator selected by a pointer-to-member
+ * (src/sim.hpp:47,51; src/sim.cpp:60-66) over TArray state (src/template_array.hpp).  This ABI
+ * is the boundary that replaces that class's compute members; the C++ host driver in
+ * gaukuk_amd/host/ keeps the reference's Sim surface (Setup, Advance, WriteData, the .in keys)
+ * and calls these entry points.  Plain C types only: pointers, sizes, doubles, ints.
+ *
+ * Conventions
+ *   ownership  the caller owns every host buffer; a context owns its device memory.
+ *   errors     every call returns gk_status; gk_last_error() describes the last failure.  No
+ *              exception crosses the ABI (the C++ wrapper rethrows, as the reference throws
+ *              std::runtime_error from its constructors, src/grid.hpp:23-25).
+ *   threading  one context per host thread; work is ordered on the context's HIP stream.
+ *   numerics   floors repair states silently, as in the reference (update_cons.cpp:55-67);
+ *              a floor hit is not an error.
+ */
+#ifndef GAUKUK_H
+#define GAUKUK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gk_ctx gk_ctx;
+
+typedef enum {
+    GK_OK = 0,
+    GK_EINVAL = 1,   /* bad argument or unsupported configuration */
+    GK_EHIP = 2,     /* a HIP runtime call failed (text in gk_last_error) */
+    GK_ENOMEM = 3,   /* device allocation failed */
+    GK_ESTATE = 4,   /* call out of order (e.g. advance before any state was uploaded) */
+    GK_ENUMERIC = 5  /* non-finite state detected by gk_check_finite */
+} gk_status;
+
+/* Grid: the keys NX, NY, NZ, NGhost, xmin..zmax of the .in file (src/grid.hpp:18-41,
+ * src/sim.cpp:16-42).  nz must be 1 (2D/1D path); nghost must be 2 (PLM stencil). */
+typedef struct {
+    int nx, ny, nz, nghost;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+} gk_grid;
+
+/* Physics and integrator: gamma (src/eos/adiabatic.cpp:12-16), CFL, integrator 1/2/3 and
+ * RcOrder 1/2 (src/sim.cpp:47,56-66), stepmax, boundary codes xleft..zright 0..3
+ * (src/boundary/boundary.cpp:13-125), constant gravity (src/source_term/source.cpp:23-31),
+ * precision 64 or 32 (the reference's Real, src/gaukuk.hpp:5-9). */
+typedef struct {
+    double gamma, cfl;
+    double g[3];
+    int integrator;
+    int rc_order;
+    int stepmax;
+    int bc[6];
+    int precision;
+} gk_phys;
+
+/* Sim::Sim() (src/sim.cpp:45-67): validate, allocate device state, select the integrator. */
+gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out);
+/* Sim::~Sim(): free everything owned by the context. */
+void gk_destroy(gk_ctx* ctx);
+
+/* State transfer in the reference TArray layout Real[5][lenz][leny][lenx] with ghosts
+ * (src/template_array.hpp:67-72; Real per gk_phys.precision).  Upload reads active cells only;
+ * download writes active cells, fills ghost cells by the boundary rules and zeroes corners and
+ * z ghost planes.  Replaces the host arrays Sim::cons is read from / written to. */
+gk_status gk_upload_cons(gk_ctx* ctx, const void* host);
+gk_status gk_download_cons(gk_ctx* ctx, void* host);
+/* Active cells only, always float64 [5][ny][nx] (converted for precision 32). */
+gk_status gk_upload_active(gk_ctx* ctx, const double* host);
+gk_status gk_download_active(gk_ctx* ctx, double* host);
+
+/* Sim::Advance(dtoutput) (src/sim.cpp:69-117): step until t reaches t + dtoutput or stepmax.
+ * On return *t and *step hold the new time and step count (either may be NULL). */
+gk_status gk_advance(gk_ctx* ctx, double dtoutput, double* t, int* step);
+/* Sim::GetTime / Getdt / GetStep and isContinue (src/sim.hpp:41-43, :26). */
+gk_status gk_get_time(gk_ctx* ctx, double* t, double* dt, int* step, int* is_continue);
+/* The per-step lines the reference prints (src/sim.cpp:100-103): the last n (<= cap) steps'
+ * (step, t, dt), oldest first.  *count receives the number written. */
+gk_status gk_step_log(gk_ctx* ctx, int* steps, double* t, double* dt, int cap, int* count);
+
+/* Enqueue exactly nsteps steps with no output deadline (for benchmarking); asynchronous. */
+gk_status gk_run_steps(gk_ctx* ctx, int nsteps);
+gk_status gk_synchronize(gk_ctx* ctx);
+/* Time nsteps steps with HIP events on the context's stream.  total_ms: whole region;
+ * kernel_ms[k] / launches[k]: summed duration and count of each kernel class
+ * k = 0 first-stage kernel, 1 combine-stage kernel, 2 clock kernels. */
+gk_status gk_profile_steps(gk_ctx* ctx, int nsteps, double* total_ms, double kernel_ms[3], int launches[3]);
+
+/* FP64 sums of the five conserved variables over active cells, deterministic. */
+gk_status gk_cons_sums(gk_ctx* ctx, double out[5]);
+/* GK_ENUMERIC if any active conserved value is NaN or infinite. */
+gk_status gk_check_finite(gk_ctx* ctx);
+
+/* Tuning: rows per wave segment of the stage kernel (0 = automatic). */
+gk_status gk_set_tile(gk_ctx* ctx, int seg_rows);
+gk_status gk_get_tile(gk_ctx* ctx, int* seg_rows, int* strips, int* waves);
+
+/* Text of the last error on this context (or of gk_create when ctx is NULL). */
+const char* gk_last_error(const gk_ctx* ctx);
+/* Library build identification (gfx target, policies). */
+const char* gk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GAUKUK_H */
