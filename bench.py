@@ -5,9 +5,8 @@ Workload: Kelvin-Helmholtz 2048x2048, FP64, PLM + HLLC + RK3, CFL 0.4, periodic,
 A step is one full RK3 step (3 fused stage kernels + 2 one-thread clock kernels) over the
 whole grid with the state resident in HBM.  Prints one JSON line:
   value         whole-job zone-updates/s over exactly --steps timed steps
-  roofline      the combine-stage kernel (the dominant one: 2 of the 3 launches per step) with
-                its algorithmic bytes (3 x 40 B per cell: read U_s, read U^n, write) divided
-                by its average launch duration from HIP events on the library's stream
+  roofline      the stage kernel with the most total time, its algorithmic bytes per launch
+                divided by its average launch duration from HIP events on the library's stream
   cpu_baseline  the oracle restatement of the reference's OpenMP CPU path (fast flags) timed on
                 this host's cores over a bounded sample of the same workload
 """
@@ -25,7 +24,12 @@ sys.path.insert(0, ROOT)
 METRIC = "zone-updates/s & %HBM roofline; time-to-L1-error learned/FP32 vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 NX = NY = 2048
-BYTES_PER_CELL = {"first": 80.0, "combine": 120.0}  # FP64, per launch
+# The three RK3 stage kernels of a step, with the algorithmic HBM bytes per cell of one launch
+# (FP64): the first stage reads U^n and writes U1 (80 B); the two combine stages read U_s and
+# U^n and write one state (120 B).
+STAGES = [("stage_kernel<double, VanLeer, THREE=false, GRAV=false, LAST=false> (RK3 stage 1)", 80.0),
+          ("stage_kernel<double, VanLeer, THREE=true, GRAV=false, LAST=false> (RK3 stage 2)", 120.0),
+          ("stage_kernel<double, VanLeer, THREE=true, GRAV=false, LAST=true> (RK3 stage 3)", 120.0)]
 
 
 def log(msg):
@@ -90,17 +94,22 @@ def main():
     assert s.check_finite()
     zones = NX * NY
     value = zones * args.steps / wall
-    comb_avg_ms = kms[1] / max(cnt[1], 1)
-    first_avg_ms = kms[0] / max(cnt[0], 1)
-    achieved = BYTES_PER_CELL["combine"] * zones / (comb_avg_ms * 1e-3) / 1e9
-    log(f"wall {wall * 1e3:.1f} ms, events {total_ms:.1f} ms; combine {comb_avg_ms * 1e3:.1f} us, "
-        f"first {first_avg_ms * 1e3:.1f} us, clock {kms[2] / max(cnt[2], 1) * 1e3:.1f} us")
+    kernels = []
+    for k, (name, bpc) in enumerate(STAGES):
+        avg_ms = kms[k] / max(cnt[k], 1)
+        kernels.append({"kernel": name, "launches": cnt[k], "avg_launch_us": avg_ms * 1e3,
+                        "bytes_per_launch": bpc * zones,
+                        "achieved_GBs": bpc * zones / (avg_ms * 1e-3) / 1e9, "total_ms": kms[k]})
+    log(f"wall {wall * 1e3:.1f} ms, events {total_ms:.1f} ms; " +
+        ", ".join(f"{q['kernel'].split('<')[1][:28]} {q['avg_launch_us']:.1f} us" for q in kernels) +
+        f"; clock kernels {kms[3] / max(cnt[3], 1) * 1e3:.1f} us")
+    dom = max(kernels, key=lambda q: q["total_ms"])
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_stage_combine_kh2048_fp64.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_stage_kernels_kh2048_fp64_rk3.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_launch", {}).get(dom["kernel"])
 
     out = {
         "metric": METRIC,
@@ -117,11 +126,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": "Kelvin-Helmholtz 2048x2048 FP64, PLM+HLLC+RK3, CFL 0.4, periodic, seeded ICs",
                    "nx": NX, "ny": NY, "integrator": "rk3", "precision": 64},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage_kernel<double,VanLeer,true> (RK3 combine stage)",
-                     "bytes_per_launch": BYTES_PER_CELL["combine"] * zones,
-                     "avg_launch_us": comb_avg_ms * 1e3},
+        "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": dom["kernel"], "bytes_per_launch": dom["bytes_per_launch"],
+                     "avg_launch_us": dom["avg_launch_us"]},
+        "stage_kernels": kernels,
         "step_roofline_frac": (320.0 * zones / (wall / args.steps)) / 1e9 / HBM_PEAK_GBS,
     }
     if not args.no_cpu:

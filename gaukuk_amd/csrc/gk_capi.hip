@@ -72,7 +72,7 @@ static gk_status fail(gk_ctx* c, gk_status s, const std::string& m) {
 static void plan_waves(gk_ctx* c, int seg_rows) {
     c->strips = (c->nx + kStrip - 1) / kStrip;
     if (seg_rows <= 0) {
-        const int target = 4096;  // ~16 waves per CU on 256 CUs
+        const int target = 3072;  // 3 waves per SIMD on 256 CUs x 4 SIMDs: one resident round
         int segs = (target + c->strips - 1) / c->strips;
         seg_rows = (c->ny + segs - 1) / segs;
         if (seg_rows < 8) seg_rows = 8;
@@ -127,17 +127,30 @@ static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
     return a;
 }
 
+using PlmLimiter = VanLeer;
+
+template <class R, class L, bool THREE>
+static void launch_stage_t(gk_ctx* c, const StageArgs<R>& a, bool last) {
+    dim3 grid(c->nblocks), block(kBlock);
+    if (c->gravity) {
+        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, false>), grid, block, 0, c->stream, a);
+    } else {
+        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, false>), grid, block, 0, c->stream, a);
+    }
+}
+
 template <class R>
 static void launch_stage(gk_ctx* c, const StageSpec& s) {
     StageArgs<R> a = make_args<R>(c, s);
-    dim3 grid(c->nblocks), block(kBlock);
     const bool plm = c->phys.rc_order != 1;
     if (s.three) {
-        if (plm) hipLaunchKernelGGL((stage_kernel<R, VanLeer, true>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, FirstOrder, true>), grid, block, 0, c->stream, a);
+        if (plm) launch_stage_t<R, PlmLimiter, true>(c, a, s.last);
+        else launch_stage_t<R, FirstOrder, true>(c, a, s.last);
     } else {
-        if (plm) hipLaunchKernelGGL((stage_kernel<R, VanLeer, false>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, FirstOrder, false>), grid, block, 0, c->stream, a);
+        if (plm) launch_stage_t<R, PlmLimiter, false>(c, a, s.last);
+        else launch_stage_t<R, FirstOrder, false>(c, a, s.last);
     }
 }
 
@@ -440,15 +453,16 @@ gk_status gk_synchronize(gk_ctx* c) {
     return GK_OK;
 }
 
-gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kernel_ms[3], int launches[3]) {
+gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kernel_ms[4], int launches[4]) {
     if (!c || nsteps < 0) return GK_EINVAL;
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    // one event before and after every launch of the step: kind 2 = clock kernels
+    // an event after every launch; the interval ending at a launch's event is its duration.
+    // classes: 0..2 the stage kernels by position in the step, 3 the clock kernels
     std::vector<hipEvent_t> ev;
     std::vector<int> kind;
-    ev.reserve((size_t)nsteps * 10 + 2);
+    ev.reserve((size_t)nsteps * 6 + 1);
     auto rec = [&]() {
         hipEvent_t e;
         (void)hipEventCreate(&e);
@@ -458,19 +472,19 @@ gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kerne
     rec();
     for (int k = 0; k < nsteps; k++) {
         hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
-        rec(); kind.push_back(2);
-        for (const StageSpec& s : c->stages) {
-            launch_stage_any(c, s);
-            rec(); kind.push_back(s.three ? 1 : 0);
+        rec(); kind.push_back(3);
+        for (size_t q = 0; q < c->stages.size(); q++) {
+            launch_stage_any(c, c->stages[q]);
+            rec(); kind.push_back((int)q);
         }
         hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap,
                            c->phys.integrator == 1 ? 1 : 0);
-        rec(); kind.push_back(2);
+        rec(); kind.push_back(3);
     }
     GK_TRY_HIP(c, hipGetLastError());
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    double kms[3] = {0, 0, 0};
-    int cnt[3] = {0, 0, 0};
+    double kms[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
     for (size_t q = 0; q + 1 < ev.size(); q++) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
@@ -481,7 +495,7 @@ gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kerne
     (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (total_ms) *total_ms = tot;
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         if (kernel_ms) kernel_ms[k] = kms[k];
         if (launches) launches[k] = cnt[k];
     }
@@ -527,6 +541,27 @@ gk_status gk_get_tile(gk_ctx* c, int* seg_rows, int* strips, int* waves) {
     if (seg_rows) *seg_rows = c->seg_rows;
     if (strips) *strips = c->strips;
     if (waves) *waves = c->nwaves;
+    return GK_OK;
+}
+
+gk_status gk_selftest(long long count, unsigned long long seed, int emax, long long* mismatches) {
+    if (count < 0 || emax < 1 || emax > 500 || !mismatches) {
+        g_create_error = "gk_selftest: bad argument";
+        return GK_EINVAL;
+    }
+    unsigned long long* d = nullptr;
+    if (hipMalloc(&d, sizeof(unsigned long long)) != hipSuccess) { g_create_error = "selftest alloc"; return GK_ENOMEM; }
+    unsigned long long h = 0;
+    gk_status st = GK_OK;
+    if (hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) st = GK_EHIP;
+    if (st == GK_OK) {
+        hipLaunchKernelGGL(selftest_kernel, dim3(1024), dim3(256), 0, 0, count, seed, emax, d);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess)
+            st = GK_EHIP;
+    }
+    (void)hipFree(d);
+    if (st != GK_OK) { g_create_error = "gk_selftest: HIP failure"; return st; }
+    *mismatches = (long long)h;
     return GK_OK;
 }
 

@@ -35,12 +35,26 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
     return (b % 8) * (nb / 8) + b / 8;
 }
 
-template <class R, class Lim, bool THREE>
-__global__ __launch_bounds__(kBlock) void stage_kernel(const StageArgs<R> a) {
+// Three waves per SIMD: the register budget (168 VGPRs) holds the three-row window, the
+// prefetched row and both Riemann solves without spills.
+#ifndef GK_WAVES_PER_EU
+#define GK_WAVES_PER_EU 3
+#endif
+
+// GRAV: constant gravity is enrolled; LAST: the stage ends the step (second half-kick, next
+// step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.
+template <class R, class Lim, bool THREE, bool GRAV, bool LAST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GK_WAVES_PER_EU)))
+void stage_kernel(const StageArgs<R> a) {
+    constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
+    constexpr bool KICK_BASE = GRAV && THREE; // the combine base is U^n
     DevClock* clk = a.clock;
     if (!clk->active) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    // the wave index is uniform across the wave: keep it (and every row index derived from
+    // it) in scalar registers so row logic is scalar branches and scalar address arithmetic
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
     if (w >= a.nwaves) return;
     const int strip = w % a.strips, seg = w / a.strips;
     const int c = strip * kStrip + lane - kHalo;
@@ -51,6 +65,7 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const StageArgs<R> a) {
     const double dt = clk->dt;
     const R dtdx = R(dt * a.dxRec), dtdy = R(dt * a.dyRec), h = R(0.5 * dt);
     const long long P = a.plane;
+    const int pitch = a.pitch;
     // forward-Euler ping-pong: the current state alternates between the two buffers
     const R* in = a.in;
     R* out = a.out;
@@ -58,129 +73,169 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const StageArgs<R> a) {
 
     int kx;
     const int cs = bc_remap(min(c, a.nx + 1), a.nx, a.bc[0], a.bc[1], kx);
-    const int cown = min(c, a.nx - 1);
+    // column of the combine operands: every lane loads them (no branch), so halo lanes clamp
+    // into [0, nx-1]; only owner lanes store
+    const int cown = min(max(c, 0), a.nx - 1);
+    // only the first and last strips hold ghost columns: the x transform is a uniform branch
+    const bool xedge = strip == 0 || strip == a.strips - 1;
 
-    R W0[5], W1[5], W2[5];  // primitive rows r-2, r-1, r (ring, renamed by the unrolled loop)
-    R pre[5];               // next input row, raw
-    R wLy[5], Gp[5];        // left state of the next y face; lower y-face flux of the update row
-#pragma unroll
-    for (int n = 0; n < 5; n++) { W0[n] = W1[n] = W2[n] = wLy[n] = Gp[n] = R(0); }
-    R cacc = R(0);
-
-    auto raw_load = [&](int r, R u[5]) {
+    // Addresses are a wave-uniform row pointer (scalar registers) plus a per-lane 32-bit byte
+    // offset, which the compiler turns into scalar-base loads and stores.
+    const long long Pb = P * (long long)sizeof(R);
+    const unsigned off_in = (unsigned)cs * (unsigned)sizeof(R);
+    const unsigned off_own = (unsigned)cown * (unsigned)sizeof(R);
+    auto at = [&](const R* row, int n, unsigned off) -> const R& {
+        return *reinterpret_cast<const R*>(reinterpret_cast<const char*>(row) + n * Pb + off);
+    };
+    // raw conserved values of row r (ghost rows remapped; the row index is wave-uniform)
+    auto load_row = [&](int r, R u[5]) {
         int ky;
         const int rs = bc_remap(r, a.ny, a.bc[2], a.bc[3], ky);
-        const R* p = in + (long long)rs * a.pitch + cs;
+        const R* row = in + (long long)rs * pitch;
 #pragma unroll
-        for (int n = 0; n < 5; n++) u[n] = p[n * P];
+        for (int n = 0; n < 5; n++) u[n] = at(row, n, off_in);
     };
-
-    const int rlast = j1 + 1;
-    raw_load(j0 - 2, pre);
-
-    auto body = [&](int r, R (&A)[5], R (&B)[5], R (&Cn)[5]) {
-        R u[5];
-#pragma unroll
-        for (int n = 0; n < 5; n++) u[n] = pre[n];
-        if (r < rlast) raw_load(r + 1, pre);
-
-        // operands of the cell update of row r-2 (owner lanes), issued before the flux math
-        const int ur = r - 2;
-        const bool upd = ur >= j0;
-        R bs[5], ut[5];
-        if (upd && owner) {
-            const long long o = (long long)ur * a.pitch + cown;
-            if (THREE) {
-#pragma unroll
-                for (int n = 0; n < 5; n++) { bs[n] = a.base[o + n * P]; ut[n] = in[o + n * P]; }
-            } else {
-#pragma unroll
-                for (int n = 0; n < 5; n++) bs[n] = in[o + n * P];
-            }
-        }
-
-        // row r: boundary transform, gravity half-kick on U^n, ConsToPrim
-        if (a.kick_in) gravity_kick(u, a.gx, a.gy, a.gz, h);
-        {
+    // gravity half-kick on U^n, boundary transforms of ghost cells, ConsToPrim
+    auto finish_row = [&](int r, R u[5], R wout[5]) {
+        if (KICK_IN) gravity_kick(u, a.gx, a.gy, a.gz, h);
+        if (xedge) bc_transform_lanes(u, 1, kx);
+        if (r < 0 || r >= a.ny) {
             int ky;
             bc_remap(r, a.ny, a.bc[2], a.bc[3], ky);
-            bc_transform(u, 1, kx);
             bc_transform(u, 2, ky);
         }
-        a.eos.cons_to_prim(u, Cn);
-
-        // y face r-1 (between rows r-2 and r-1): left state from the slope of row r-2 (kept in
-        // wLy), right state from the slope of row r-1.
-        R G[5];
-        if (r >= j0) {
-            R wR[5], wLn[5];
+        a.eos.cons_to_prim(u, wout);
+    };
+    // y slope of row B (between A below and C above): right state of B's lower face, left state
+    // of its upper face
+    auto yslope = [&](const R A[5], const R B[5], const R C[5], R wR[5], R wL[5]) {
 #pragma unroll
-            for (int n = 0; n < 5; n++) {
-                const R s = Lim::slope(B[n] - A[n], Cn[n] - B[n]);
-                wR[n] = B[n] - R(0.5) * s;
-                wLn[n] = B[n] + R(0.5) * s;
-            }
-            if (r >= j0 + 1) {
-                // direction VLY rotation (src/flux/hllc.cpp:25-27): (den, vy, vz, vx, p)
-                const R l[5] = {wLy[0], wLy[2], wLy[3], wLy[1], wLy[4]};
-                const R rr[5] = {wR[0], wR[2], wR[3], wR[1], wR[4]};
-                R f[5];
-                RiemannPolicy::flux(a.eos, l, rr, f);
-                G[0] = f[0]; G[2] = f[1]; G[3] = f[2]; G[1] = f[3]; G[4] = f[4];
-            }
-#pragma unroll
-            for (int n = 0; n < 5; n++) wLy[n] = wLn[n];
+        for (int n = 0; n < 5; n++) {
+            const R s = Lim::slope(B[n] - A[n], C[n] - B[n]);
+            wR[n] = B[n] - R(0.5) * s;
+            wL[n] = B[n] + R(0.5) * s;
         }
-
-        // update of row r-2 (primitives in A): x faces c and c+1 through neighbour lanes
-        if (upd) {
-            R F[5], Fn[5];
-            {
-                R wR[5], wLo[5], wL[5];
+    };
+    // y-face flux with the direction-VLY rotation (src/flux/hllc.cpp:25-27): (den, vy, vz, vx, p)
+    auto yflux = [&](const R wl[5], const R wr[5], R G[5]) {
+        const R l[5] = {wl[0], wl[2], wl[3], wl[1], wl[4]};
+        const R rr[5] = {wr[0], wr[2], wr[3], wr[1], wr[4]};
+        R f[5];
+        RiemannPolicy::flux(a.eos, l, rr, f);
+        G[0] = f[0]; G[2] = f[1]; G[3] = f[2]; G[1] = f[3]; G[4] = f[4];
+    };
+    // x-face fluxes of one row: face c (F) and face c+1 (Fn) through neighbour lanes
+    auto xflux = [&](const R A[5], R F[5], R Fn[5]) {
+        R wR[5], wLo[5], wL[5];
 #pragma unroll
-                for (int n = 0; n < 5; n++) {
-                    const R am = __shfl_up(A[n], 1), ap = __shfl_down(A[n], 1);
-                    const R s = Lim::slope(A[n] - am, ap - A[n]);
-                    wR[n] = A[n] - R(0.5) * s;   // right state of face c
-                    wLo[n] = A[n] + R(0.5) * s;  // left state of face c+1
-                }
-#pragma unroll
-                for (int n = 0; n < 5; n++) wL[n] = __shfl_up(wLo[n], 1);
-                RiemannPolicy::flux(a.eos, wL, wR, F);
-#pragma unroll
-                for (int n = 0; n < 5; n++) Fn[n] = __shfl_down(F[n], 1);
-            }
-            if (owner) {
-                // the base of every stage is U^n after the first gravity half-kick
-                if (THREE ? a.kick_base : a.kick_in) gravity_kick(bs, a.gx, a.gy, a.gz, h);
-                R res[5];
-#pragma unroll
-                for (int n = 0; n < 5; n++) {
-                    const R dF = (Fn[n] - F[n]) * dtdx + (G[n] - Gp[n]) * dtdy;
-                    if (THREE) res[n] = a.c1 * bs[n] + a.c2 * ut[n] - a.c3 * dF;
-                    else res[n] = a.c1 * bs[n] - a.c2 * dF;
-                }
-                a.eos.floors(res);
-                if (a.kick_out) gravity_kick(res, a.gx, a.gy, a.gz, h);
-                const long long o = (long long)ur * a.pitch + c;
-#pragma unroll
-                for (int n = 0; n < 5; n++) out[o + n * P] = res[n];
-                if (a.want_cmax) cacc = smax(cacc, a.eos.wave_speed(res));
-            }
+        for (int n = 0; n < 5; n++) {
+            const R am = __shfl_up(A[n], 1), ap = __shfl_down(A[n], 1);
+            const R s = Lim::slope(A[n] - am, ap - A[n]);
+            wR[n] = A[n] - R(0.5) * s;   // right state of face c
+            wLo[n] = A[n] + R(0.5) * s;  // left state of face c+1
         }
-        if (r >= j0 + 1) {
 #pragma unroll
-            for (int n = 0; n < 5; n++) Gp[n] = G[n];
+        for (int n = 0; n < 5; n++) wL[n] = __shfl_up(wLo[n], 1);
+        RiemannPolicy::flux(a.eos, wL, wR, F);
+#pragma unroll
+        for (int n = 0; n < 5; n++) Fn[n] = __shfl_down(F[n], 1);
+    };
+
+    R pre[5];               // next input row, raw
+    R wLy[5], Gp[5];        // left state of the next y face; y flux at the lower face of row r-2
+    R cacc = R(0);
+    const int rlast = j1 + 1;
+
+    // the update of row r-2 from its x fluxes (F, Fn) and y fluxes (Gp below, G above)
+    auto update = [&](int r, const R bs[5], const R ut[5], const R F[5], const R Fn[5], const R G[5]) {
+        const long long o = (long long)(r - 2) * pitch;
+        R b2[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) b2[n] = bs[n];
+        // the base of every stage is U^n after the first gravity half-kick
+        if (KICK_IN || KICK_BASE) gravity_kick(b2, a.gx, a.gy, a.gz, h);
+        R res[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            const R dF = (Fn[n] - F[n]) * dtdx + (G[n] - Gp[n]) * dtdy;
+            if (THREE) res[n] = a.c1 * b2[n] + a.c2 * ut[n] - a.c3 * dF;
+            else res[n] = a.c1 * b2[n] - a.c2 * dF;
+        }
+        a.eos.floors(res);
+        if (GRAV && LAST) gravity_kick(res, a.gx, a.gy, a.gz, h);
+        if (LAST) {
+            const R ws = a.eos.wave_speed(res);
+            cacc = owner ? smax(cacc, ws) : cacc;
+        }
+        if (owner) {
+            char* row = reinterpret_cast<char*>(out + o) + off_own;
+#pragma unroll
+            for (int n = 0; n < 5; n++) *reinterpret_cast<R*>(row + n * Pb) = res[n];
+        }
+    };
+    // operands of the update of row r-2, issued before the flux math
+    auto load_operands = [&](int r, R bs[5], R ut[5]) {
+        const long long o = (long long)(r - 2) * pitch;
+        if (THREE) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) { bs[n] = at(a.base + o, n, off_own); ut[n] = at(in + o, n, off_own); }
+        } else {
+#pragma unroll
+            for (int n = 0; n < 5; n++) { bs[n] = at(in + o, n, off_own); ut[n] = R(0); }
         }
     };
 
-    for (int r = j0 - 2;;) {
+    R W0[5], W1[5], W2[5];  // primitive rows, rotated by the unrolled loop
+    {
+        R u[5];
+        load_row(j0 - 2, pre);
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = pre[n];
+        load_row(j0 - 1, pre);
+        finish_row(j0 - 2, u, W0);
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = pre[n];
+        load_row(j0, pre);
+        finish_row(j0 - 1, u, W1);
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = pre[n];
+        load_row(j0 + 1, pre);
+        finish_row(j0, u, W2);
+        R wR[5];
+        yslope(W0, W1, W2, wR, wLy);   // wLy: left state of face j0
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = pre[n];
+        load_row(j0 + 2, pre);
+        finish_row(j0 + 1, u, W0);     // W0 now holds row j0+1
+        R wLn[5];
+        yslope(W1, W2, W0, wR, wLn);   // slope of row j0
+        yflux(wLy, wR, Gp);            // face j0
+#pragma unroll
+        for (int n = 0; n < 5; n++) wLy[n] = wLn[n];
+    }
+    // steady state at newest row r: y face r-1 and the update of row r-2 (primitives in A)
+    auto body = [&](int r, R (&A)[5], R (&B)[5], R (&Cn)[5]) {
+        R u[5], bs[5], ut[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = pre[n];
+        load_row(min(r + 1, rlast), pre);
+        load_operands(r, bs, ut);
+        finish_row(r, u, Cn);
+        R wR[5], wLn[5], G[5], F[5], Fn[5];
+        yslope(A, B, Cn, wR, wLn);
+        yflux(wLy, wR, G);
+        xflux(A, F, Fn);
+        update(r, bs, ut, F, Fn, G);
+#pragma unroll
+        for (int n = 0; n < 5; n++) { Gp[n] = G[n]; wLy[n] = wLn[n]; }
+    };
+    for (int r = j0 + 2;;) {
+        body(r, W2, W0, W1); if (++r > rlast) break;
         body(r, W0, W1, W2); if (++r > rlast) break;
         body(r, W1, W2, W0); if (++r > rlast) break;
-        body(r, W2, W0, W1); if (++r > rlast) break;
     }
 
-    if (a.want_cmax) {
+    if (LAST) {
         const double m = wave_max((double)cacc);
         if (lane == 0) atomicMax(&clk->cmax_bits[clk->parity ^ 1], dbits(m));
     }
@@ -271,6 +326,37 @@ __global__ void sums_final_kernel(const double* partial, int nblocks, double* ou
         for (int b = 0; b < nblocks; b++) s += partial[b * 5 + threadIdx.x];
         out[threadIdx.x] = s;
     }
+}
+
+// Self-check of the short FP64 quotient and root (qdiv, qsqrt) against the compiler's IEEE
+// operators on operands drawn from a SplitMix64 stream: mantissas uniform, exponents uniform in
+// [-emax, emax] (quotient operands) and [-2*emax, 2*emax] (root operand), random signs for the
+// quotient.  Counts every result whose bits differ.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double draw(unsigned long long bits, int emax, bool sign) {
+    const unsigned long long mant = bits & 0xFFFFFFFFFFFFFull;
+    const int e = (int)((bits >> 52) % (unsigned long long)(2 * emax + 1)) - emax;
+    unsigned long long v = ((unsigned long long)(e + 1023) << 52) | mant;
+    if (sign && ((bits >> 63) & 1ull)) v |= 1ull << 63;
+    return __longlong_as_double((long long)v);
+}
+__global__ void selftest_kernel(long long n, unsigned long long seed, int emax, unsigned long long* bad) {
+    unsigned long long local = 0;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        const double a = draw(mix64(seed ^ (3ull * k)), emax, true);
+        const double b = draw(mix64(seed ^ (3ull * k + 1)), emax, true);
+        const double x = fabs(draw(mix64(seed ^ (3ull * k + 2)), 2 * emax, false));
+        const double q_ieee = a / b, q = qdiv(a, b);
+        const double r_ieee = __builtin_sqrt(x), r = qsqrt(x);
+        local += (__double_as_longlong(q) != __double_as_longlong(q_ieee));
+        local += (__double_as_longlong(r) != __double_as_longlong(r_ieee));
+    }
+    if (local) atomicAdd(bad, local);
 }
 
 }  // namespace gk

@@ -3,7 +3,7 @@
 // The reference selects its equation of state and Riemann solver by linking one source file
 // each (Makefile:64-90, configure.py:8-19: EOS=adiabatic, FLUX=hllc).  Here the same choice is
 // a template argument of the stage kernel: `--eos` / `--flux` at configure time set GK_EOS /
-// GK_FLUX, which name the policy types below (gk_select.hpp).  Every policy is
+// GK_FLUX, which name the policy types below (gk_device.hpp).  Every policy is
 // __host__ __device__ so host code and kernels share one definition.
 //
 // Arithmetic follows the reference's operation order so that the FP64 path differs from the
@@ -22,6 +22,47 @@ template <class R> __host__ __device__ __forceinline__ R smax(R a, R b) { return
 template <class R> __host__ __device__ __forceinline__ R gsqrt(R x);
 template <> __host__ __device__ __forceinline__ double gsqrt<double>(double x) { return __builtin_sqrt(x); }
 template <> __host__ __device__ __forceinline__ float gsqrt<float>(float x) { return __builtin_sqrtf(x); }
+
+// FP64 quotient and square root for operands in the normal range.  The IEEE sequences the
+// compiler emits wrap a core of (reciprocal, two Newton steps, residual correction) for the
+// quotient and (reciprocal square root, Goldschmidt/Newton refinement) for the root in
+// range-scaling and special-value steps (div_scale/div_fixup, ldexp and class tests).  For
+// normal finite operands whose quotient is normal those steps are the identity, so the cores
+// below return the same bits with 3 (quotient) and 8 (root) fewer instructions.  gk_selftest()
+// checks the identity on the device.  The host build uses the plain operators.
+__host__ __device__ __forceinline__ double qdiv(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = n * r;
+    const double rem = __builtin_fma(-d, q, n);
+    return __builtin_fma(rem, r, q);
+#else
+    return n / d;
+#endif
+}
+__host__ __device__ __forceinline__ float qdiv(float n, float d) { return n / d; }
+
+__host__ __device__ __forceinline__ double qsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+#else
+    return __builtin_sqrt(x);
+#endif
+}
+__host__ __device__ __forceinline__ float qsqrt(float x) { return __builtin_sqrtf(x); }
 
 template <class R> __host__ __device__ __forceinline__ R gabs(R x) { return x < R(0) ? -x : x; }
 
@@ -47,13 +88,13 @@ struct Adiabatic {
         e.gmRec = R(1.0) / e.gamma;
         return e;
     }
-    __host__ __device__ __forceinline__ R sound_speed(R d, R p) const { return gsqrt(gamma * p / d); }
+    __host__ __device__ __forceinline__ R sound_speed(R d, R p) const { return qsqrt(qdiv(gamma * p, d)); }
     __host__ __device__ __forceinline__ R egas(R, R p) const { return p * gm1Rec; }
 
     // ConsToPrim with the floors clamped on local copies: adiabatic.cpp:18-61.
     __host__ __device__ __forceinline__ void cons_to_prim(const R u[5], R w[5]) const {
         R d = smax(u[0], R(GK_DENSITY_FLOOR));
-        R inv = R(1.0) / d;
+        R inv = qdiv(R(1.0), d);
         w[0] = d;
         w[1] = u[1] * inv;
         w[2] = u[2] * inv;
@@ -66,30 +107,34 @@ struct Adiabatic {
     // One cell's max_d |v_d| + c_s with CalCmax's own kinetic-energy form: adiabatic.cpp:115-160.
     __host__ __device__ __forceinline__ R wave_speed(const R u[5]) const {
         R d = smax(u[0], R(GK_DENSITY_FLOOR));
-        R inv = R(1.0) / d;
+        R inv = qdiv(R(1.0), d);
         R v1 = u[1] * inv, v2 = u[2] * inv, v3 = u[3] * inv;
         R ek = R(0.5) * (v1 * u[1] + v2 * u[2] + v3 * u[3]);
         R p = gm1 * (u[4] - ek);
         p = (p > R(GK_PRESSURE_FLOOR)) ? p : R(GK_PRESSURE_FLOOR);
-        R cs = gsqrt(gamma * p * inv);
+        R cs = qsqrt(gamma * p * inv);
         R loc = gabs(v1) + cs;
         loc = smax(loc, gabs(v2) + cs);
         loc = smax(loc, gabs(v3) + cs);
         return loc;
     }
 
-    // Post-update floors of UpdateCons: src/evolution/update_cons.cpp:55-67.
+    // Post-update floors of UpdateCons: src/evolution/update_cons.cpp:55-67, written without
+    // branches (both outcomes are computed and selected; the quotient 1/r[0] of the low-density
+    // case is discarded).  pm_gm1 = pmin/gm1 is the same quotient the reference forms in place.
     __host__ __device__ __forceinline__ void floors(R r[5]) const {
         const R dmin = R(GK_DENSITY_FLOOR), pmin = R(GK_PRESSURE_FLOOR);
-        if (r[0] < dmin) {
-            r[0] = dmin; r[1] = R(0.0); r[2] = R(0.0); r[3] = R(0.0);
-            r[4] = pmin / gm1 + R(0.0);
-        } else {
-            R inv = R(1.0) / r[0];
-            R ke = R(0.5) * inv * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
-            R press = gm1 * (r[4] - ke);
-            if (press < pmin) r[4] = pmin / gm1 + ke;
-        }
+        const R pm_gm1 = pmin / gm1;
+        const bool low_d = r[0] < dmin;
+        const R inv = qdiv(R(1.0), r[0]);
+        const R ke = R(0.5) * inv * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+        const R press = gm1 * (r[4] - ke);
+        const R e_fix = (press < pmin) ? pm_gm1 + ke : r[4];
+        r[4] = low_d ? pm_gm1 + R(0.0) : e_fix;
+        r[1] = low_d ? R(0.0) : r[1];
+        r[2] = low_d ? R(0.0) : r[2];
+        r[3] = low_d ? R(0.0) : r[3];
+        r[0] = low_d ? dmin : r[0];
     }
 };
 
@@ -101,7 +146,7 @@ struct VanLeer {
     static constexpr int order = 2;
     template <class R> __host__ __device__ __forceinline__ static R slope(R a, R b) {
         R multi = a * b;
-        R sigma = R(2.0) * multi / (a + b);
+        R sigma = qdiv(R(2.0) * multi, a + b);
         return (multi <= R(0.0)) ? R(0.0) : sigma;
     }
 };
@@ -114,8 +159,7 @@ struct FirstOrder {
 // HLLC Riemann solver (Toro ch. 10): src/flux/hllc.cpp:22-103.
 // wl, wr: primitive states rotated so index 1 is the face-normal velocity and 2, 3 the
 // tangential ones in the reference's cyclic order (hllc.cpp:25-27); f receives the flux in the
-// same rotated order.  The left/right blend is a multiply, as in the reference, so a NaN on the
-// unselected side propagates.
+// same rotated order.
 struct Hllc {
     template <class R, class EOS>
     __host__ __device__ __forceinline__ static void flux(const EOS& eos, const R wl[5], const R wr[5], R f[5]) {
@@ -126,31 +170,32 @@ struct Hllc {
         R pv = R(0.5) * (prel + prer) - R(0.125) * (vxr - vxl) * (denl + denr) * (al + ar);
         pv = smax(pv, R(0));
         const R qk = R(0.5) * (R(1) + eos.gmRec);
-        R tmp = smax(pv / prel - R(1), R(0));
-        const R ql = gsqrt(R(1) + qk * tmp);
-        tmp = smax(pv / prer - R(1), R(0));
-        const R qr = gsqrt(R(1) + qk * tmp);
+        R tmp = smax(qdiv(pv, prel) - R(1), R(0));
+        const R ql = qsqrt(R(1) + qk * tmp);
+        tmp = smax(qdiv(pv, prer) - R(1), R(0));
+        const R qr = qsqrt(R(1) + qk * tmp);
         const R sl = vxl - al * ql;
         const R sr = vxr + ar * qr;
-        const R ss = (prer - prel + denl * vxl * (sl - vxl) - denr * vxr * (sr - vxr)) /
-                     (denl * (sl - vxl) - denr * (sr - vxr));
-        const R el = eos.egas(denl, prel) + R(0.5) * denl * (vxl * vxl + vyl * vyl + vzl * vzl);
-        const R er = eos.egas(denr, prer) + R(0.5) * denr * (vxr * vxr + vyr * vyr + vzr * vzr);
-        const R cl = smin(R(0), sl) * (ss - vxl) / (sl - ss);
-        const R cr = smax(R(0), sr) * (ss - vxr) / (sr - ss);
-        const R sel = (ss >= R(0)) ? R(1.0) : R(0.0);
-        const R ser = R(1.0) - sel;
-        const R fl1 = denl * (vxl + cl);
-        const R fl2 = denl * (vxl * vxl + cl * sl) + prel;
-        const R fl5 = (el + prel) * (vxl + cl) + denl * cl * ss * (sl - vxl);
-        const R fr1 = denr * (vxr + cr);
-        const R fr2 = denr * (vxr * vxr + cr * sr) + prer;
-        const R fr5 = (er + prer) * (vxr + cr) + denr * cr * ss * (sr - vxr);
-        f[0] = sel * fl1 + ser * fr1;
-        f[1] = sel * fl2 + ser * fr2;
-        f[2] = sel * (fl1 * vyl) + ser * (fr1 * vyr);
-        f[3] = sel * (fl1 * vzl) + ser * (fr1 * vzr);
-        f[4] = sel * fl5 + ser * fr5;
+        const R ss = qdiv(prer - prel + denl * vxl * (sl - vxl) - denr * vxr * (sr - vxr),
+                          denl * (sl - vxl) - denr * (sr - vxr));
+        // Only the upwind side of the contact enters the flux: the reference blends
+        // F = sel*F_L + (1-sel)*F_R with sel in {0,1} (hllc.cpp:64-83), which equals the
+        // selected side's flux whenever the other side is finite.
+        const bool left = ss >= R(0);
+        const R dk = left ? denl : denr, vk = left ? vxl : vxr, v2 = left ? vyl : vyr;
+        const R v3 = left ? vzl : vzr, pk = left ? prel : prer, sk = left ? sl : sr;
+        // left: smin(0, sl) = (sl < 0 ? sl : 0); right: smax(0, sr) = (0 < sr ? sr : 0)
+        // (bitwise logic: a short-circuit form becomes a branch)
+        const bool take = (left & (sk < R(0))) | ((!left) & (R(0) < sk));
+        const R mk = take ? sk : R(0);
+        const R ek = eos.egas(dk, pk) + R(0.5) * dk * (vk * vk + v2 * v2 + v3 * v3);
+        const R ck = qdiv(mk * (ss - vk), sk - ss);
+        const R f1 = dk * (vk + ck);
+        f[0] = f1;
+        f[1] = dk * (vk * vk + ck * sk) + pk;
+        f[2] = f1 * v2;
+        f[3] = f1 * v3;
+        f[4] = (ek + pk) * (vk + ck) + dk * ck * ss * (sk - vk);
     }
 };
 
@@ -177,6 +222,24 @@ __host__ __device__ __forceinline__ void bc_transform(R u[5], int comp, int kind
         u[comp] = mc;
         u[4] = eint + keNew;
     }
+}
+
+// The same transforms without branches, for a kind that differs between lanes of a wave
+// (ghost columns): every outcome is computed and the right one selected.
+template <class R>
+__host__ __device__ __forceinline__ void bc_transform_lanes(R u[5], int comp, int kind) {
+    const R m = u[comp];
+    const R mc = (kind == 2) ? smin(m, R(0.0)) : smax(m, R(0.0));
+    const R d = smax(u[0], R(GK_DENSITY_FLOOR));
+    const R inv = R(1.0) / d;
+    const R keOld = R(0.5) * inv * m * m;
+    const R keNew = R(0.5) * inv * mc * mc;
+    R eint = u[4] - keOld;
+    eint = smax(eint, R(1e-16));
+    const bool outflow = kind >= 2;
+    u[4] = outflow ? eint + keNew : u[4];
+    u[comp] = outflow ? mc : ((kind == 1) ? -m : m);
+    u[0] = outflow ? d : u[0];
 }
 
 // Ghost index map for one axis of length n with face types (lo, hi): returns the source index

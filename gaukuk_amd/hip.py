@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgaukuk_hip.so")
+LIB_PATH = os.environ.get("GK_LIB_PATH") or os.path.join(HERE, "libgaukuk_hip.so")
 
 GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ESTATE", 5: "GK_ENUMERIC"}
 
@@ -20,7 +20,7 @@ GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ES
 SYMBOLS = ["gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
-           "gk_get_tile", "gk_last_error", "gk_version"]
+           "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version"]
 
 
 class GkGrid(C.Structure):
@@ -83,6 +83,8 @@ def lib() -> C.CDLL:
         L.gk_get_tile.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]
         L.gk_last_error.restype = C.c_char_p
         L.gk_last_error.argtypes = [vp]
+        L.gk_selftest.restype = st
+        L.gk_selftest.argtypes = [C.c_longlong, C.c_ulonglong, ip, C.POINTER(C.c_longlong)]
         L.gk_version.restype = C.c_char_p
         L.gk_version.argtypes = []
         _lib = L
@@ -209,8 +211,8 @@ class Solver:
 
     def profile(self, n: int):
         tot = C.c_double()
-        kms = (C.c_double * 3)()
-        cnt = (C.c_int * 3)()
+        kms = (C.c_double * 4)()
+        cnt = (C.c_int * 4)()
         self._check(self.L.gk_profile_steps(self.h, n, C.byref(tot), kms, cnt))
         return tot.value, list(kms), list(cnt)
 
@@ -230,6 +232,15 @@ class Solver:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         self._check(self.L.gk_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+
+def selftest(count: int = 1 << 24, seed: int = 12345, emax: int = 500) -> int:
+    """Bitwise check of the kernels' short FP64 quotient/root against IEEE; returns mismatches."""
+    bad = C.c_longlong()
+    st = lib().gk_selftest(count, seed, emax, C.byref(bad))
+    if st != 0:
+        raise GaukukError(st, lib().gk_last_error(None).decode())
+    return bad.value
 
 
 def version() -> str:

@@ -85,10 +85,10 @@ gk_status gk_step_log(gk_ctx* ctx, int* steps, double* t, double* dt, int cap, i
 /* Enqueue exactly nsteps steps with no output deadline (for benchmarking); asynchronous. */
 gk_status gk_run_steps(gk_ctx* ctx, int nsteps);
 gk_status gk_synchronize(gk_ctx* ctx);
-/* Time nsteps steps with HIP events on the context's stream.  total_ms: whole region;
- * kernel_ms[k] / launches[k]: summed duration and count of each kernel class
- * k = 0 first-stage kernel, 1 combine-stage kernel, 2 clock kernels. */
-gk_status gk_profile_steps(gk_ctx* ctx, int nsteps, double* total_ms, double kernel_ms[3], int launches[3]);
+/* Time nsteps steps with HIP events on the context's stream (one event after every launch).
+ * total_ms: the whole region; kernel_ms[k] / launches[k]: summed duration and count of the
+ * stage kernel at position k = 0, 1, 2 within the step, and k = 3 the one-thread clock kernels. */
+gk_status gk_profile_steps(gk_ctx* ctx, int nsteps, double* total_ms, double kernel_ms[4], int launches[4]);
 
 /* FP64 sums of the five conserved variables over active cells, deterministic. */
 gk_status gk_cons_sums(gk_ctx* ctx, double out[5]);
@@ -98,6 +98,12 @@ gk_status gk_check_finite(gk_ctx* ctx);
 /* Tuning: rows per wave segment of the stage kernel (0 = automatic). */
 gk_status gk_set_tile(gk_ctx* ctx, int seg_rows);
 gk_status gk_get_tile(gk_ctx* ctx, int* seg_rows, int* strips, int* waves);
+
+/* Device self-check, no counterpart in the reference: the short FP64 quotient and square-root
+ * sequences of the stage kernels against the IEEE operators on `count` operand draws with
+ * binary exponents in [-emax, emax] (root operands [-2*emax, 2*emax]); *mismatches counts results
+ * whose bits differ.  Expected 0 for emax <= 500. */
+gk_status gk_selftest(long long count, unsigned long long seed, int emax, long long* mismatches);
 
 /* Text of the last error on this context (or of gk_create when ctx is NULL). */
 const char* gk_last_error(const gk_ctx* ctx);
