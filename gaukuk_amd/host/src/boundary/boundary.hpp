@@ -1,0 +1,42 @@
+// boundary.hpp -- boundary-condition dispatch.  As in the reference (src/boundary/boundary.hpp:
+// 15-117, boundary.cpp:13-125) each face gets a function chosen by its type code from the
+// configuration: 0 copy the edge cell, 1 periodic, 2 reflective, 3 outflow (mirror, inflow
+// momentum clamped, energy adjusted); unknown codes fall back to 0.  On the device the same codes
+// select an index remap applied while the stage kernel loads its halo, so no ghost cells are
+// stored; the host functions here fill ghosts of host arrays (initial state and output files).
+#pragma once
+
+#include <functional>
+
+#include "../eos/eos.hpp"
+#include "../gaukuk.hpp"
+#include "../grid.hpp"
+#include "../template_array.hpp"
+
+namespace Gaukuk {
+
+class Boundary {
+public:
+    using BoundaryFunc = std::function<void(TArray<Real>&, const Grid&, const EquationOfState&)>;
+    Boundary();
+    BoundaryFunc Bdxl, Bdxr, Bdyl, Bdyr, Bdzl, Bdzr;
+    // type codes per face (xleft, xright, yleft, yright, zleft, zright) as passed to the device
+    int type[6];
+    bool selfDefined[6] = {false, false, false, false, false, false};
+
+    void UpdateBD(TArray<Real>& cons, const Grid& grid, const Domain& domain, const EquationOfState& eos);
+
+    // Self-defined faces (setup-file hooks, src/boundary/boundary.hpp:28-45).  The device path has
+    // no user ghost-fill kernel: Sim::Advance rejects a run with a self-defined face.
+    void EnrollSelfDefineBDXL(BoundaryFunc f) { Bdxl = std::move(f); selfDefined[0] = true; }
+    void EnrollSelfDefineBDXR(BoundaryFunc f) { Bdxr = std::move(f); selfDefined[1] = true; }
+    void EnrollSelfDefineBDYL(BoundaryFunc f) { Bdyl = std::move(f); selfDefined[2] = true; }
+    void EnrollSelfDefineBDYR(BoundaryFunc f) { Bdyr = std::move(f); selfDefined[3] = true; }
+    void EnrollSelfDefineBDZL(BoundaryFunc f) { Bdzl = std::move(f); selfDefined[4] = true; }
+    void EnrollSelfDefineBDZR(BoundaryFunc f) { Bdzr = std::move(f); selfDefined[5] = true; }
+    bool AnySelfDefined() const;
+
+    static BoundaryFunc ForType(int axis, bool high, int code);
+};
+
+}  // namespace Gaukuk

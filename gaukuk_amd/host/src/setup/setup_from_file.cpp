@@ -1,0 +1,40 @@
+// setup_from_file.cpp -- active-region conserved state from the binary file named by
+// $GK_IC_FILE: int32 nvar (5), int32 nx, int32 ny, float64 cons[5][ny][nx]; config keys
+// gx, gy, gz enroll constant gravity.  Lets any array (e.g. from gaukuk_amd.problems) start a run.
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../gaukuk.hpp"
+#include "../sim.hpp"
+#include "../utils/utils.hpp"
+
+namespace Gaukuk {
+
+void Sim::Setup() {
+    const char* path = std::getenv("GK_IC_FILE");
+    if (!path) throw std::runtime_error("GK_IC_FILE is not set");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    int hdr[3] = {0, 0, 0};
+    const bool ok = std::fread(hdr, sizeof(int), 3, f) == 3;
+    if (!ok || hdr[0] != NVar || hdr[1] != grid.nx || hdr[2] != grid.ny) {
+        std::fclose(f);
+        throw std::runtime_error("IC file header does not match NX/NY");
+    }
+    const size_t n = (size_t)NVar * grid.nx * grid.ny;
+    std::vector<double> buf(n);
+    const size_t got = std::fread(buf.data(), sizeof(double), n, f);
+    std::fclose(f);
+    if (got != n) throw std::runtime_error("short IC data");
+    for (int v = 0; v < NVar; v++)
+        for (int j = 0; j < grid.ny; j++)
+            for (int i = 0; i < grid.nx; i++)
+                cons(v, grid.kb, grid.jb + j, grid.ib + i) = (Real)buf[((size_t)v * grid.ny + j) * grid.nx + i];
+    Config& c = Config::getInstance();
+    srcTerm.EnrollConstGravityVector(c.get("gx", 0.0), c.get("gy", 0.0), c.get("gz", 0.0));
+}
+
+}  // namespace Gaukuk

@@ -72,6 +72,33 @@ def run_reference(prob, workdir):
     return last.active().copy(), float(last.t), steps, len(outs) - 1
 
 
+def output_files():
+    """The reference's t = 0 output (after Setup and UpdateBD) of a small mixed-boundary case in
+    both data types: pins the byte layout of cons_/prim_ files including ghosts, corners and the
+    empty z planes (src/utils/write_sim.cpp:14-58, src/main.cpp:24-40)."""
+    prob = P.blast(20, 16, p_ratio=30.0, radius=0.12, center=(0.4, 0.45), bc=(3, 0, 2, 1))
+    rho, mx, my, mz, e = prob.cons
+    prob.cons = P.prim_to_cons(rho, np.full_like(rho, 0.3), np.full_like(rho, -0.2), 0 * rho,
+                               e * (prob.gamma - 1.0), prob.gamma)
+    files = {}
+    for dtype in (0, 1):
+        with tempfile.TemporaryDirectory() as d:
+            ic = os.path.join(d, "ic.bin")
+            with open(ic, "wb") as f:
+                np.array([5, prob.nx, prob.ny], np.int32).tofile(f)
+                np.ascontiguousarray(prob.cons, np.float64).tofile(f)
+            vals = prob.with_(tmax=0.0, dtoutput=0.1).in_values()
+            vals["dataType"] = dtype
+            write_in(os.path.join(d, "case.in"), vals)
+            env = dict(os.environ, GK_IC_FILE=ic, OMP_NUM_THREADS="2")
+            subprocess.run([os.path.join(REF, "gaukuk_ref"), "-i", "case.in"], cwd=d, env=env,
+                           capture_output=True, check=True)
+            name = "prim_00000" if dtype else "cons_00000"
+            files[name] = np.fromfile(os.path.join(d, name), np.uint8)
+            files["in_" + name] = np.frombuffer(open(os.path.join(d, "case.in"), "rb").read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "out_blast20x16_t0.npz"), ic=prob.cons, **files)
+
+
 def vectors(rng):
     def ref_op(op, payload_ints, arrays, n_cfg, nout):
         with tempfile.TemporaryDirectory() as d:
@@ -136,6 +163,8 @@ def main():
         print(f"{name:28s} {prob.nx}x{prob.ny} steps={steps} t={t!r}")
     vectors(np.random.default_rng(20251203))
     print("vectors written")
+    output_files()
+    print("output files written")
 
 
 if __name__ == "__main__":
