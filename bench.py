@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -61,6 +63,91 @@ def cpu_baseline(prob, budget_s: float = 12.0):
                       f"setup excluded"}
 
 
+def bench_ensemble(args):
+    """512 seeded 2D blast problems, 64^2..512^2, FP32, RK2, each with its own dt: aggregate
+    zone-updates/s of the batched persistent launches (eager and graph-replayed) against one
+    single-grid launch per problem per stage, on the same data."""
+    from gaukuk_amd import autotune, hip
+    from gaukuk_amd import problems as P
+    log("building 512-problem ensemble")
+    probs = P.ensemble(512, seed=0)
+    ens = hip.Ensemble(probs, precision=32)
+    cells = ens.cells
+    log(f"{len(probs)} problems, {cells} cells; autotuning")
+    best, table = autotune.tune_ensemble(ens, steps=2)
+    log(f"tile (seg_rows, waves/CU) = {best}; table {table}")
+    for q, p in enumerate(probs):
+        ens.upload(q, p.cons)
+    ens.run_steps(args.warmup)
+    ens.synchronize()
+    K = args.steps
+    ms_batched = ens.time(K, hip.BATCHED)
+    gsteps = 10 if K % 10 == 0 else 1
+    ens.capture(gsteps)
+    ens.run_steps(gsteps, hip.GRAPH)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ms_graph = ens.time(K, hip.GRAPH)
+    wall_graph = time.perf_counter() - t0
+    kl = max(2, K // 10)
+    ms_looped = ens.time(kl, hip.LOOPED)
+    zu = lambda ms, n: cells * n / (ms * 1e-3)
+    per_step_bytes = 100.0 * cells  # FP32 RK2: stage 1 reads + writes 20 B each, stage 2 reads 40 B, writes 20 B
+    out = {
+        "metric": METRIC, "value": zu(ms_graph, K), "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": K, "warmup": args.warmup, "ms_per_step": ms_graph / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "ensemble of 512 seeded 2D blast problems, 64..512 per side, FP32, "
+                               "PLM+HLLC+RK2, CFL 0.4, periodic or outflow, one persistent launch per stage, "
+                               "graph-captured", "problems": len(probs), "cells": cells,
+                   "tile": list(best)},
+        "batched_eager_zone_updates_per_s": zu(ms_batched, K),
+        "graph_zone_updates_per_s": zu(ms_graph, K),
+        "looped_single_grid_zone_updates_per_s": zu(ms_looped, kl),
+        "speedup_vs_looped": zu(ms_graph, K) / zu(ms_looped, kl),
+        "step_hbm_frac": per_step_bytes / (ms_graph / K * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "wall_s_graph": wall_graph,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_rt(args):
+    """Rayleigh-Taylor 1024x3072 with gravity, RK2, to t = 0.2 in FP64 and in FP32 (FP64 dt and
+    conservation sums in both): zone-updates/s of each and the relative L1 of FP32 vs FP64."""
+    from gaukuk_amd import hip
+    from gaukuk_amd import problems as P
+    prob = P.rayleigh_taylor(1024, 3072, tmax=0.2, dtoutput=0.2)
+    res = {}
+    for prec in (64, 32):
+        warm = hip.Solver(prob, precision=prec)
+        warm.run_steps(args.warmup)
+        warm.synchronize()
+        warm.close()
+        s = hip.Solver(prob, precision=prec)
+        sums0 = s.sums()
+        t0 = time.perf_counter()
+        s.run_to_end()
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        res[prec] = dict(state=s.download(), steps=s.step, t=s.t, wall=wall, sums0=sums0, sums1=s.sums())
+        log(f"RT FP{prec}: {s.step} steps to t={s.t:.6f} in {wall:.3f} s")
+    a, b = res[32]["state"], res[64]["state"]
+    l1 = [float(np.abs(a[n] - b[n]).sum() / max(np.abs(b[n]).sum(), 1e-300)) for n in range(5)]
+    zu = {p: prob.nx * prob.ny * res[p]["steps"] / res[p]["wall"] for p in res}
+    out = {
+        "metric": METRIC, "value": zu[64], "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": res[64]["steps"], "warmup": args.warmup, "ms_per_step": res[64]["wall"] * 1e3 / res[64]["steps"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "Rayleigh-Taylor 1024x3072, gravity -0.1, periodic x / reflecting y, PLM+HLLC+RK2, "
+                               "CFL 0.4, to t = 0.2", "nx": 1024, "ny": 3072},
+        "fp32_zone_updates_per_s": zu[32], "fp32_steps": res[32]["steps"],
+        "fp32_vs_fp64_rel_l1": l1,
+        "mass_drift_fp64": float(res[64]["sums1"][0] / res[64]["sums0"][0] - 1),
+        "mass_drift_fp32": float(res[32]["sums1"][0] / res[32]["sums0"][0] - 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +155,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seg-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "rt"],
+                    help="kh: the headline (default); ensemble / rt: the other benchmark configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU (--gpus 1): runs across several GPUs are not part of this build",
@@ -76,6 +165,11 @@ def main():
 
     from gaukuk_amd import hip
     from gaukuk_amd import problems as P
+
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
+    if args.workload == "rt":
+        return bench_rt(args)
 
     log("building KH 2048^2 initial conditions")
     prob = P.kelvin_helmholtz(NX, seed=0, integrator=3, stepmax=-1)

@@ -572,3 +572,5 @@ const char* gk_version(void) {
 }
 
 }  // extern "C"
+
+#include "gk_ensemble.hip"

@@ -43,20 +43,14 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 
 // GRAV: constant gravity is enrolled; LAST: the stage ends the step (second half-kick, next
 // step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.
+// One wave advances the cells of strip `strip`, segment `seg` of one grid by one stage.
+// `strip` and `seg` are wave-uniform (scalar registers), so row logic is scalar branches and
+// scalar address arithmetic.
 template <class R, class Lim, bool THREE, bool GRAV, bool LAST>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GK_WAVES_PER_EU)))
-void stage_kernel(const StageArgs<R> a) {
+__device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane) {
     constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
     constexpr bool KICK_BASE = GRAV && THREE; // the combine base is U^n
     DevClock* clk = a.clock;
-    if (!clk->active) return;
-    const int lane = threadIdx.x & (kWave - 1);
-    // the wave index is uniform across the wave: keep it (and every row index derived from
-    // it) in scalar registers so row logic is scalar branches and scalar address arithmetic
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
-    if (w >= a.nwaves) return;
-    const int strip = w % a.strips, seg = w / a.strips;
     const int c = strip * kStrip + lane - kHalo;
     const int j0 = seg * a.seg_rows;
     const int j1 = min(j0 + a.seg_rows, a.ny);
@@ -241,6 +235,81 @@ void stage_kernel(const StageArgs<R> a) {
     }
 }
 
+// One RK stage of one grid: wave w of the (XCD-remapped) grid takes strip w % strips, segment
+// w / strips.
+template <class R, class Lim, bool THREE, bool GRAV, bool LAST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GK_WAVES_PER_EU)))
+void stage_kernel(const StageArgs<R> a) {
+    if (!a.clock->active) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
+    if (w >= a.nwaves) return;
+    march<R, Lim, THREE, GRAV, LAST>(a, w % a.strips, w / a.strips, lane);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ensembles: many independent grids of uneven sizes advanced together.  Each problem's stage is
+// split into work items (strip, segment); a persistent grid of waves claims items from an atomic
+// counter until all are done, so large and small problems share the machine without a launch per
+// problem.  Each problem has its own clock (its own dt).
+struct EnsProblem {
+    long long off[3];     // element offsets of the problem's state buffers A, B, C in the arena
+    long long plane;
+    int pitch, nx, ny, seg_rows, strips, items, item_base;
+    int bc[4];
+    double dxRec, dyRec;
+};
+
+template <class R>
+struct EnsArgs {
+    R* arena;
+    const EnsProblem* probs;
+    DevClock* clocks;
+    int* counter;         // work-queue head of this stage (reset by ens_step_begin_kernel)
+    int nprob, total_items;
+    int in_buf, base_buf, out_buf;
+    StageArgs<R> proto;   // stage constants (coefficients, flags, gravity, EOS)
+};
+
+template <class R, class Lim, bool THREE, bool GRAV, bool LAST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GK_WAVES_PER_EU)))
+void ens_stage_kernel(const EnsArgs<R> e) {
+    const int lane = threadIdx.x & (kWave - 1);
+    int item = 0;
+    if (lane == 0) item = atomicAdd(e.counter, 1);
+    item = __builtin_amdgcn_readfirstlane(item);
+    while (item < e.total_items) {
+        // claim the next item before working on this one, to hide the atomic's latency
+        int next = 0;
+        if (lane == 0) next = atomicAdd(e.counter, 1);
+        next = __builtin_amdgcn_readfirstlane(next);
+        // the problem owning `item`: last p with item_base <= item (binary search, scalar loads)
+        int lo = 0, hi = e.nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (e.probs[mid].item_base <= item) lo = mid; else hi = mid - 1;
+        }
+        const EnsProblem& d = e.probs[lo];
+        DevClock* clk = e.clocks + lo;
+        if (clk->active) {
+            StageArgs<R> a = e.proto;
+            a.in = e.arena + d.off[e.in_buf];
+            a.base = e.arena + d.off[e.base_buf];
+            a.out = e.arena + d.off[e.out_buf];
+            a.plane = d.plane;
+            a.pitch = d.pitch; a.nx = d.nx; a.ny = d.ny;
+            a.seg_rows = d.seg_rows; a.strips = d.strips;
+            a.bc[0] = d.bc[0]; a.bc[1] = d.bc[1]; a.bc[2] = d.bc[2]; a.bc[3] = d.bc[3];
+            a.dxRec = d.dxRec; a.dyRec = d.dyRec;
+            a.clock = clk;
+            const int local = item - d.item_base;
+            march<R, Lim, THREE, GRAV, LAST>(a, local % d.strips, local / d.strips, lane);
+        }
+        item = next;
+    }
+}
+
 // Max wave speed of a state (step-0 dt, or after an upload): CalCmax, adiabatic.cpp:115-160.
 template <class R>
 __global__ __launch_bounds__(kBlock) void cmax_kernel(const R* u, long long plane, int pitch, int nx, int ny,
@@ -268,7 +337,7 @@ __global__ void advance_begin_kernel(DevClock* c, double dtoutput) {
 }
 
 // Loop test and dt of one step (src/sim.cpp:72-81).  cmax_bits[parity] holds max(1e-16, cells).
-__global__ void step_begin_kernel(DevClock* c) {
+__device__ __forceinline__ void clock_begin(DevClock* c) {
     if (!c->active) return;
     const double gap = c->t_next - c->t;
     if (!((gap < 0 ? -gap : gap) > 1e-12 && c->is_continue)) { c->active = 0; return; }
@@ -279,18 +348,47 @@ __global__ void step_begin_kernel(DevClock* c) {
     c->cmax_bits[c->parity ^ 1] = dbits(1e-16);
 }
 
-// Loop tail (src/sim.cpp:93-98) plus the step log entry (the reference prints it, :100-103).
-__global__ void step_end_kernel(DevClock* c, StepRecord* log, int cap, int toggle_cur) {
-    if (!c->active) return;
+// Loop tail (src/sim.cpp:93-98).
+__device__ __forceinline__ bool clock_end(DevClock* c, int toggle_cur) {
+    if (!c->active) return false;
     c->t += c->dt;
     c->step++;
     c->dt_until_output = c->t_next - c->t;
     if (c->stepmax > 0 && c->step >= c->stepmax) c->is_continue = 0;
+    c->parity ^= 1;
+    if (toggle_cur) c->cur ^= 1;
+    return true;
+}
+
+__global__ void step_begin_kernel(DevClock* c) { clock_begin(c); }
+
+// plus the step log entry (the reference prints it, src/sim.cpp:100-103)
+__global__ void step_end_kernel(DevClock* c, StepRecord* log, int cap, int toggle_cur) {
+    if (!clock_end(c, toggle_cur)) return;
     StepRecord rec;
     rec.t = c->t; rec.dt = c->dt; rec.step = c->step; rec.pad = 0;
     log[(c->step - 1) % cap] = rec;
-    c->parity ^= 1;
-    if (toggle_cur) c->cur ^= 1;
+}
+
+// Ensemble step prologue/epilogue: one thread per problem; the first thread also rewinds the
+// work-queue heads of the step's stages (a kernel, not a memset, so the step can be captured).
+__global__ void ens_step_begin_kernel(DevClock* clocks, int n, int* counters, int ncounters) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < ncounters) counters[p] = 0;
+    if (p < n) clock_begin(clocks + p);
+}
+__global__ void ens_step_end_kernel(DevClock* clocks, int n, int toggle_cur) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) clock_end(clocks + p, toggle_cur);
+}
+__global__ void ens_advance_begin_kernel(DevClock* clocks, int n, double dtoutput) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) {
+        DevClock* c = clocks + p;
+        c->t_next = c->t + dtoutput;
+        c->dt_until_output = dtoutput;
+        c->active = c->is_continue;
+    }
 }
 
 // Deterministic FP64 sums of the five conserved variables: fixed cell->thread assignment and

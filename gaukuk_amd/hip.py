@@ -20,7 +20,10 @@ GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ES
 SYMBOLS = ["gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
-           "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version"]
+           "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version",
+           "gk_ens_create", "gk_ens_destroy", "gk_ens_upload", "gk_ens_download", "gk_ens_set_tile",
+           "gk_ens_get_tile", "gk_ens_run_steps", "gk_ens_capture", "gk_ens_synchronize", "gk_ens_time",
+           "gk_ens_get_time", "gk_ens_last_error"]
 
 
 class GkGrid(C.Structure):
@@ -87,6 +90,28 @@ def lib() -> C.CDLL:
         L.gk_selftest.argtypes = [C.c_longlong, C.c_ulonglong, ip, C.POINTER(C.c_longlong)]
         L.gk_version.restype = C.c_char_p
         L.gk_version.argtypes = []
+        L.gk_ens_create.restype = st
+        L.gk_ens_create.argtypes = [ip, C.POINTER(GkGrid), C.POINTER(GkPhys), C.POINTER(vp)]
+        L.gk_ens_destroy.argtypes = [vp]
+        for name in ("gk_ens_upload", "gk_ens_download"):
+            getattr(L, name).restype = st
+            getattr(L, name).argtypes = [vp, ip, dp]
+        L.gk_ens_set_tile.restype = st
+        L.gk_ens_set_tile.argtypes = [vp, ip, ip]
+        L.gk_ens_get_tile.restype = st
+        L.gk_ens_get_tile.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]
+        L.gk_ens_run_steps.restype = st
+        L.gk_ens_run_steps.argtypes = [vp, ip, ip]
+        L.gk_ens_capture.restype = st
+        L.gk_ens_capture.argtypes = [vp, ip]
+        L.gk_ens_synchronize.restype = st
+        L.gk_ens_synchronize.argtypes = [vp]
+        L.gk_ens_time.restype = st
+        L.gk_ens_time.argtypes = [vp, ip, ip, dp]
+        L.gk_ens_get_time.restype = st
+        L.gk_ens_get_time.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
+        L.gk_ens_last_error.restype = C.c_char_p
+        L.gk_ens_last_error.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -232,6 +257,92 @@ class Solver:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         self._check(self.L.gk_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+
+BATCHED, LOOPED, GRAPH = 0, 1, 2
+
+
+class Ensemble:
+    """Many independent problems advanced together on the GPU (gk_ens_* entry points).
+
+    Problems share precision, integrator, RcOrder, gamma and gravity; each has its own size,
+    extent, CFL, stepmax, boundaries and time step."""
+
+    def __init__(self, probs, precision: int = 32, seg_rows: int = 0, waves_per_cu: int = 0):
+        self.L = lib()
+        self.probs = list(probs)
+        P = len(self.probs)
+        grids = (GkGrid * P)(*[make_grid(p) for p in self.probs])
+        phys = (GkPhys * P)(*[make_phys(p, precision) for p in self.probs])
+        h = C.c_void_p()
+        st = self.L.gk_ens_create(P, grids, phys, C.byref(h))
+        if st != 0:
+            raise GaukukError(st, self.L.gk_ens_last_error(None).decode())
+        self.h = h
+        self.precision = precision
+        if seg_rows or waves_per_cu:
+            self.set_tile(seg_rows, waves_per_cu)
+        for q, p in enumerate(self.probs):
+            self.upload(q, p.cons)
+
+    def _check(self, st):
+        if st != 0:
+            raise GaukukError(st, self.L.gk_ens_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gk_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.probs)
+
+    @property
+    def cells(self) -> int:
+        return sum(p.nx * p.ny for p in self.probs)
+
+    def upload(self, q: int, cons):
+        a = np.ascontiguousarray(cons, np.float64)
+        self._check(self.L.gk_ens_upload(self.h, q, _dp(a)))
+
+    def download(self, q: int) -> np.ndarray:
+        p = self.probs[q]
+        out = np.empty((5, p.ny, p.nx), np.float64)
+        self._check(self.L.gk_ens_download(self.h, q, _dp(out)))
+        return out
+
+    def set_tile(self, seg_rows: int = 0, waves_per_cu: int = 0):
+        self._check(self.L.gk_ens_set_tile(self.h, seg_rows, waves_per_cu))
+
+    def tile(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.L.gk_ens_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def run_steps(self, n: int, mode: int = BATCHED):
+        self._check(self.L.gk_ens_run_steps(self.h, n, mode))
+
+    def capture(self, n: int):
+        self._check(self.L.gk_ens_capture(self.h, n))
+
+    def synchronize(self):
+        self._check(self.L.gk_ens_synchronize(self.h))
+
+    def time(self, n: int, mode: int = BATCHED) -> float:
+        ms = C.c_double()
+        self._check(self.L.gk_ens_time(self.h, n, mode, C.byref(ms)))
+        return ms.value
+
+    def get_time(self, q: int):
+        t, dt, s = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.L.gk_ens_get_time(self.h, q, C.byref(t), C.byref(dt), C.byref(s)))
+        return t.value, dt.value, s.value
 
 
 def selftest(count: int = 1 << 24, seed: int = 12345, emax: int = 500) -> int:

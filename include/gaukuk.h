@@ -105,6 +105,33 @@ gk_status gk_get_tile(gk_ctx* ctx, int* seg_rows, int* strips, int* waves);
  * whose bits differ.  Expected 0 for emax <= 500. */
 gk_status gk_selftest(long long count, unsigned long long seed, int emax, long long* mismatches);
 
+/* ---- Ensembles: many independent grids advanced together (no counterpart in the reference,
+ * which runs one problem per process; each problem follows Sim::Advance semantics with its own
+ * dt).  Problems share precision, integrator, RcOrder, gamma and gravity; sizes, extents, CFL,
+ * stepmax and boundary codes are per problem.  A step runs one clock kernel for all problems, one
+ * persistent launch per RK stage over every problem's (strip, segment) work items, and one clock
+ * kernel for the t update. */
+typedef struct gk_ens gk_ens;
+gk_status gk_ens_create(int count, const gk_grid* grids, const gk_phys* phys, gk_ens** out);
+void gk_ens_destroy(gk_ens* ens);
+/* Active cells of problem p, float64 [5][ny][nx] (converted for precision 32). */
+gk_status gk_ens_upload(gk_ens* ens, int p, const double* host);
+gk_status gk_ens_download(gk_ens* ens, int p, double* host);
+/* Tile: rows per segment (0 = default 16) and resident waves per CU of the persistent grid. */
+gk_status gk_ens_set_tile(gk_ens* ens, int seg_rows, int waves_per_cu);
+gk_status gk_ens_get_tile(gk_ens* ens, int* seg_rows, int* waves_per_cu, int* total_items);
+/* Enqueue nsteps steps without an output deadline.  mode 0: batched persistent launches;
+ * mode 1: one single-grid launch per problem per stage (the looped baseline); mode 2: replay the
+ * captured graph (nsteps a multiple of the captured step count). */
+gk_status gk_ens_run_steps(gk_ens* ens, int nsteps, int mode);
+/* Capture nsteps batched steps into a HIP graph for mode 2. */
+gk_status gk_ens_capture(gk_ens* ens, int nsteps);
+gk_status gk_ens_synchronize(gk_ens* ens);
+/* Run nsteps in `mode` between two HIP events on the ensemble's stream; *ms = elapsed time. */
+gk_status gk_ens_time(gk_ens* ens, int nsteps, int mode, double* ms);
+gk_status gk_ens_get_time(gk_ens* ens, int p, double* t, double* dt, int* step);
+const char* gk_ens_last_error(const gk_ens* ens);
+
 /* Text of the last error on this context (or of gk_create when ctx is NULL). */
 const char* gk_last_error(const gk_ctx* ctx);
 /* Library build identification (gfx target, policies). */
