@@ -3,6 +3,7 @@
 // A context owns three state buffers (SoA planes of the active region), the device clock, a
 // step log and reduction scratch.  Integrators are tables of stage launches selected once at
 // creation, as the reference selects its pointer-to-member integrator (src/sim.cpp:60-66).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,7 +54,16 @@ struct gk_ctx {
     int bc[4] = {0, 0, 0, 0};
     std::vector<StageSpec> stages;
     std::string err;
+    // learned flux correction (gk_learned.hip)
+    gk_net* net = nullptr;                 // attached network: stages use the corrected kernel
+    unsigned long long* lstats = nullptr;  // corrected / sensor / positivity face counts
+    double* acc = nullptr;                 // recorded RK-weighted dt * flux per face
+    bool recording = false;
+    std::vector<double> rec_w;             // each stage's share of the step's flux
 };
+
+static bool learned_launch(gk_ctx* c, const StageSpec& s);
+static void record_launch(gk_ctx* c, size_t q);
 
 #define GK_TRY_HIP(ctx, call)                                                              \
     do {                                                                                   \
@@ -155,13 +165,17 @@ static void launch_stage(gk_ctx* c, const StageSpec& s) {
 }
 
 static void launch_stage_any(gk_ctx* c, const StageSpec& s) {
+    if (c->net && learned_launch(c, s)) return;
     if (c->prec == 32) launch_stage<float>(c, s);
     else launch_stage<double>(c, s);
 }
 
 static void enqueue_step(gk_ctx* c) {
     hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
-    for (const StageSpec& s : c->stages) launch_stage_any(c, s);
+    for (size_t q = 0; q < c->stages.size(); q++) {
+        if (c->recording) record_launch(c, q);
+        launch_stage_any(c, c->stages[q]);
+    }
     hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap,
                        c->phys.integrator == 1 ? 1 : 0);
 }
@@ -332,6 +346,8 @@ void gk_destroy(gk_ctx* c) {
     if (c->log) (void)hipFree(c->log);
     if (c->partial) (void)hipFree(c->partial);
     if (c->sums) (void)hipFree(c->sums);
+    if (c->acc) (void)hipFree(c->acc);
+    if (c->lstats) (void)hipFree(c->lstats);
     if (c->hclock) (void)hipHostFree(c->hclock);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -574,3 +590,4 @@ const char* gk_version(void) {
 }  // extern "C"
 
 #include "gk_ensemble.hip"
+#include "gk_learned.hip"

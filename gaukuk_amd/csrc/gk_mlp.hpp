@@ -1,0 +1,141 @@
+// gk_mlp.hpp -- the flux-correction MLP (16 -> 64 -> 64 -> 4, ReLU) on CDNA4 matrix cores.
+//
+// No counterpart in the reference.  Every product is v_mfma_f32_16x16x16_bf16: BF16 operands,
+// FP32 accumulation.  The network runs in transposed form, H^T = W X^T, 16 samples per tile:
+//   A operand (16 x 16, rows x k): lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
+//   B operand (16 x 16, k x cols): lane l holds B[4 (l / 16) + i][l % 16]
+//   D result  (16 x 16):           lane l holds D[4 (l / 16) + i][l % 16]
+// so the D tile of one layer (rows = hidden units, columns = samples) is already the B operand of
+// the next: no data moves between layers.  Weights are the A operands, read once per wave.  The
+// weight gradients contract over samples, which needs each tile transposed (through LDS).
+//
+// Parameter vector (FP32 master copy, 5508 values): W1[64][16], b1[64], W2[64][64], b2[64],
+// W3[4][64], b3[4], row-major (W[out][in]).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gk {
+namespace mlp {
+
+constexpr int kIn = 16, kHid = 64, kOut = 4;
+constexpr int oW1 = 0, ob1 = 1024, oW2 = 1088, ob2 = 5184, oW3 = 5248, ob3 = 5504, kParams = 5508;
+// LDS row pitches in BF16 elements: rows 40 / 136 bytes apart, so the 16 rows an A-operand read
+// touches fall in distinct bank pairs.
+constexpr int kP16 = 20;
+constexpr int kP64 = 68;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bfloat16, round to nearest, ties to even (finite inputs).
+__host__ __device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+
+__device__ __forceinline__ bf16x4 pack4(f32x4 v) {
+    bf16x4 r;
+    r[0] = (short)f2bf(v[0]); r[1] = (short)f2bf(v[1]); r[2] = (short)f2bf(v[2]); r[3] = (short)f2bf(v[3]);
+    return r;
+}
+__device__ __forceinline__ f32x4 relu(f32x4 v) {
+    f32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) r[i] = v[i] > 0.f ? v[i] : 0.f;
+    return r;
+}
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x4 lds4(const unsigned short* p) {
+    return *reinterpret_cast<const bf16x4*>(p);
+}
+
+// Network weights in LDS: BF16 matrices (padded rows), FP32 biases and the fixed input/output
+// scales (network input = feature * in_scale; correction = output * out_scale).
+struct Smem {
+    unsigned short w1[kHid * kP16];
+    unsigned short w2[kHid * kP64];
+    unsigned short w3[kOut * kP64];
+    float b1[kHid], b2[kHid], b3[kOut];
+    float in_scale[kIn], out_scale[kOut];
+};
+// Backward operands: W2^T and W3^T (rows padded to 16 outputs with zeros).
+struct SmemT {
+    unsigned short w2t[kHid * kP64];
+    unsigned short w3t[kHid * kP16];
+};
+
+__device__ __forceinline__ void stage(Smem& s, const float* p, const float* in_scale, const float* out_scale,
+                                      int tid, int nt) {
+    for (int k = tid; k < kHid * kIn; k += nt) s.w1[(k / kIn) * kP16 + k % kIn] = f2bf(p[oW1 + k]);
+    for (int k = tid; k < kHid * kHid; k += nt) s.w2[(k / kHid) * kP64 + k % kHid] = f2bf(p[oW2 + k]);
+    for (int k = tid; k < kOut * kHid; k += nt) s.w3[(k / kHid) * kP64 + k % kHid] = f2bf(p[oW3 + k]);
+    for (int k = tid; k < kHid; k += nt) { s.b1[k] = p[ob1 + k]; s.b2[k] = p[ob2 + k]; }
+    for (int k = tid; k < kOut; k += nt) { s.b3[k] = p[ob3 + k]; s.out_scale[k] = out_scale[k]; }
+    for (int k = tid; k < kIn; k += nt) s.in_scale[k] = in_scale[k];
+}
+__device__ __forceinline__ void stage_t(SmemT& s, const float* p, int tid, int nt) {
+    for (int k = tid; k < kHid * kHid; k += nt) {
+        const int o = k / kHid, i = k % kHid;   // W2[o][i] -> w2t[i][o]
+        s.w2t[i * kP64 + o] = f2bf(p[oW2 + k]);
+    }
+    for (int k = tid; k < kHid * 16; k += nt) {
+        const int h = k / 16, o = k % 16;       // W3[o][h] -> w3t[h][o], o >= 4 zero
+        s.w3t[h * kP16 + o] = o < kOut ? f2bf(p[oW3 + o * kHid + h]) : (unsigned short)0;
+    }
+}
+
+// A operands and biases of one wave's lane, loaded once.
+struct Regs {
+    bf16x4 w1[4];      // [rt]: W1[16 rt + r][4 g .. 4 g + 3]
+    bf16x4 w2[4][4];   // [ro][kb]: W2[16 ro + r][16 kb + 4 g .. + 3]
+    bf16x4 w3[4];      // [kb]: r < 4 ? W3[r][16 kb + 4 g .. + 3] : 0
+    f32x4 b1[4], b2[4], b3;
+};
+
+__device__ __forceinline__ void load_regs(Regs& w, const Smem& s, int lane) {
+    const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        w.w1[t] = lds4(&s.w1[(16 * t + r) * kP16 + 4 * g]);
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) w.w2[t][kb] = lds4(&s.w2[(16 * t + r) * kP64 + 16 * kb + 4 * g]);
+        const bf16x4 z = {0, 0, 0, 0};
+        w.w3[t] = r < kOut ? lds4(&s.w3[r * kP64 + 16 * t + 4 * g]) : z;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            w.b1[t][i] = s.b1[16 * t + 4 * g + i];
+            w.b2[t][i] = s.b2[16 * t + 4 * g + i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) w.b3[i] = g == 0 ? s.b3[i] : 0.f;
+}
+
+// Forward pass of one 16-sample tile.  x: lane (g, s) holds the scaled inputs 4g..4g+3 of sample
+// s as BF16.  Returns the D tile of the output layer: lanes 0..15 hold outputs 0..3 of sample
+// `lane`; other lanes hold zeros.
+__device__ __forceinline__ f32x4 forward_tile(const Regs& w, bf16x4 x) {
+    bf16x4 h1[4], h2[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) h1[t] = pack4(relu(mfma(w.w1[t], x, w.b1[t])));
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        f32x4 z = w.b2[t];
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) z = mfma(w.w2[t][kb], h1[kb], z);
+        h2[t] = pack4(relu(z));
+    }
+    f32x4 y = w.b3;
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) y = mfma(w.w3[kb], h2[kb], y);
+    return y;
+}
+
+}  // namespace mlp
+}  // namespace gk

@@ -23,7 +23,12 @@ SYMBOLS = ["gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_
            "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version",
            "gk_ens_create", "gk_ens_destroy", "gk_ens_upload", "gk_ens_download", "gk_ens_set_tile",
            "gk_ens_get_tile", "gk_ens_run_steps", "gk_ens_capture", "gk_ens_synchronize", "gk_ens_time",
-           "gk_ens_get_time", "gk_ens_last_error"]
+           "gk_ens_get_time", "gk_ens_last_error",
+           "gk_net_create", "gk_net_destroy", "gk_net_set_params", "gk_net_get_params", "gk_net_set_scales",
+           "gk_net_get_scales", "gk_net_set_kappa", "gk_net_get_kappa", "gk_net_forward", "gk_net_gradient", "gk_net_adam", "gk_net_reserve",
+           "gk_net_add_samples", "gk_net_dataset", "gk_net_dataset_stats", "gk_net_train", "gk_net_last_error",
+           "gk_net_collect", "gk_flux_record", "gk_flux_download", "gk_restrict", "gk_set_net", "gk_net_stats"]
+NET_PARAMS = 5508
 
 
 class GkGrid(C.Structure):
@@ -112,6 +117,50 @@ def lib() -> C.CDLL:
         L.gk_ens_get_time.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
         L.gk_ens_last_error.restype = C.c_char_p
         L.gk_ens_last_error.argtypes = [vp]
+        fp, ll, llp = C.POINTER(C.c_float), C.c_longlong, C.POINTER(C.c_longlong)
+        L.gk_net_create.restype = st
+        L.gk_net_create.argtypes = [C.POINTER(vp)]
+        L.gk_net_destroy.argtypes = [vp]
+        for name in ("gk_net_set_params", "gk_net_get_params"):
+            getattr(L, name).restype = st
+            getattr(L, name).argtypes = [vp, fp]
+        for name in ("gk_net_set_scales", "gk_net_get_scales"):
+            getattr(L, name).restype = st
+            getattr(L, name).argtypes = [vp, fp, fp]
+        L.gk_net_set_kappa.restype = st
+        L.gk_net_set_kappa.argtypes = [vp, C.c_float]
+        L.gk_net_get_kappa.restype = st
+        L.gk_net_get_kappa.argtypes = [vp, fp]
+        L.gk_net_forward.restype = st
+        L.gk_net_forward.argtypes = [vp, ll, fp, fp]
+        L.gk_net_gradient.restype = st
+        L.gk_net_gradient.argtypes = [vp, ll, fp, fp, fp, fp, dp]
+        L.gk_net_adam.restype = st
+        L.gk_net_adam.argtypes = [vp, fp, C.c_float]
+        L.gk_net_reserve.restype = st
+        L.gk_net_reserve.argtypes = [vp, ll]
+        L.gk_net_add_samples.restype = st
+        L.gk_net_add_samples.argtypes = [vp, ll, fp, fp, fp]
+        L.gk_net_dataset.restype = st
+        L.gk_net_dataset.argtypes = [vp, ll, ll, fp, fp, fp, llp]
+        L.gk_net_dataset_stats.restype = st
+        L.gk_net_dataset_stats.argtypes = [vp, dp, dp, dp]
+        L.gk_net_train.restype = st
+        L.gk_net_train.argtypes = [vp, ip, ip, ip, C.c_float, C.c_ulonglong, fp]
+        L.gk_net_last_error.restype = C.c_char_p
+        L.gk_net_last_error.argtypes = [vp]
+        L.gk_net_collect.restype = st
+        L.gk_net_collect.argtypes = [vp, vp, vp, dp, llp]
+        L.gk_flux_record.restype = st
+        L.gk_flux_record.argtypes = [vp, ip]
+        L.gk_flux_download.restype = st
+        L.gk_flux_download.argtypes = [vp, dp]
+        L.gk_restrict.restype = st
+        L.gk_restrict.argtypes = [vp, vp]
+        L.gk_set_net.restype = st
+        L.gk_set_net.argtypes = [vp, vp]
+        L.gk_net_stats.restype = st
+        L.gk_net_stats.argtypes = [vp, llp, ip]
         _lib = L
     return _lib
 
@@ -257,6 +306,169 @@ class Solver:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         self._check(self.L.gk_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    # learned flux correction
+    def flux_record(self, on: bool = True):
+        self._check(self.L.gk_flux_record(self.h, 1 if on else 0))
+
+    def n_faces(self) -> int:
+        nx, ny = self.prob.nx, self.prob.ny
+        return (nx + 1) * ny + nx * (ny + 1)
+
+    def flux_download(self):
+        """Recorded sums as (fx [4, ny, nx+1], fy [4, ny+1, nx]): mass, m_x, m_y, E."""
+        nx, ny = self.prob.nx, self.prob.ny
+        acc = np.empty((4, self.n_faces()))
+        self._check(self.L.gk_flux_download(self.h, _dp(acc)))
+        nxf = (nx + 1) * ny
+        return acc[:, :nxf].reshape(4, ny, nx + 1), acc[:, nxf:].reshape(4, ny + 1, nx)
+
+    def restrict_from(self, fine: "Solver"):
+        self._check(self.L.gk_restrict(fine.h, self.h))
+
+    def set_net(self, net):
+        self._check(self.L.gk_set_net(self.h, net.h if net is not None else None))
+        self._net = net   # keep the network alive while attached
+
+    def net_stats(self, reset: bool = False):
+        out = (C.c_longlong * 3)()
+        self._check(self.L.gk_net_stats(self.h, out, 1 if reset else 0))
+        return list(out)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Net:
+    """The flux-correction MLP on the GPU: parameters, Adam state and a device data set."""
+
+    def __init__(self, params=None, in_scale=None, out_scale=None):
+        self.L = lib()
+        h = C.c_void_p()
+        st = self.L.gk_net_create(C.byref(h))
+        if st != 0:
+            raise GaukukError(st, "gk_net_create failed")
+        self.h = h
+        if params is not None:
+            self.set_params(params)
+        if in_scale is not None or out_scale is not None:
+            self.set_scales(np.ones(16) if in_scale is None else in_scale,
+                            np.ones(4) if out_scale is None else out_scale)
+
+    def _check(self, st):
+        if st != 0:
+            raise GaukukError(st, self.L.gk_net_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gk_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32)
+        assert a.shape == (NET_PARAMS,)
+        self._check(self.L.gk_net_set_params(self.h, _fp(a)))
+
+    def params(self) -> np.ndarray:
+        a = np.empty(NET_PARAMS, np.float32)
+        self._check(self.L.gk_net_get_params(self.h, _fp(a)))
+        return a
+
+    def set_scales(self, in_scale, out_scale):
+        i = np.ascontiguousarray(in_scale, np.float32)
+        o = np.ascontiguousarray(out_scale, np.float32)
+        self._check(self.L.gk_net_set_scales(self.h, _fp(i), _fp(o)))
+
+    def scales(self):
+        i, o = np.empty(16, np.float32), np.empty(4, np.float32)
+        self._check(self.L.gk_net_get_scales(self.h, _fp(i), _fp(o)))
+        return i, o
+
+    @property
+    def kappa(self) -> float:
+        k = C.c_float()
+        self._check(self.L.gk_net_get_kappa(self.h, C.byref(k)))
+        return k.value
+
+    @kappa.setter
+    def kappa(self, value: float):
+        self._check(self.L.gk_net_set_kappa(self.h, value))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((len(x), 4), np.float32)
+        self._check(self.L.gk_net_forward(self.h, len(x), _fp(x), _fp(y)))
+        return y
+
+    def gradient(self, x, t, wt=None):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        wt = np.ascontiguousarray(np.ones(len(x)) if wt is None else wt, np.float32)
+        g = np.empty(NET_PARAMS, np.float32)
+        loss = C.c_double()
+        self._check(self.L.gk_net_gradient(self.h, len(x), _fp(x), _fp(t), _fp(wt), _fp(g), C.byref(loss)))
+        return loss.value, g
+
+    def adam(self, grad, lr: float):
+        g = np.ascontiguousarray(grad, np.float32)
+        self._check(self.L.gk_net_adam(self.h, _fp(g), lr))
+
+    def reserve(self, cap: int):
+        self._check(self.L.gk_net_reserve(self.h, cap))
+
+    def add_samples(self, x, t, wt=None):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        wt = np.ascontiguousarray(np.ones(len(x)) if wt is None else wt, np.float32)
+        self._check(self.L.gk_net_add_samples(self.h, len(x), _fp(x), _fp(t), _fp(wt)))
+
+    def size(self) -> int:
+        n = C.c_longlong()
+        self._check(self.L.gk_net_dataset(self.h, 0, 0, None, None, None, C.byref(n)))
+        return n.value
+
+    def dataset(self, start: int = 0, count: int = -1):
+        n = self.size()
+        count = n - start if count < 0 else count
+        x = np.empty((count, 16), np.float32)
+        t = np.empty((count, 4), np.float32)
+        w = np.empty(count, np.float32)
+        self._check(self.L.gk_net_dataset(self.h, start, count, _fp(x), _fp(t), _fp(w), None))
+        return x, t, w
+
+    def stats(self):
+        mx, mt, w = np.zeros(16), np.zeros(4), C.c_double()
+        self._check(self.L.gk_net_dataset_stats(self.h, _dp(mx), _dp(mt), C.byref(w)))
+        return mx, mt, w.value
+
+    def train(self, iters: int, lr: float, blocks: int = 256, tiles_per_wave: int = 1, seed: int = 0):
+        log = np.empty(max(iters, 1), np.float32)
+        self._check(self.L.gk_net_train(self.h, iters, blocks, tiles_per_wave, lr, seed, _fp(log)))
+        return log[:iters]
+
+    def collect(self, fine: Solver, coarse: Solver):
+        dt, added = C.c_double(), C.c_longlong()
+        self._check(self.L.gk_net_collect(self.h, fine.h, coarse.h, C.byref(dt), C.byref(added)))
+        return dt.value, added.value
+
+    def save(self, path: str, **meta):
+        i, o = self.scales()
+        np.savez(path, params=self.params(), in_scale=i, out_scale=o, kappa=self.kappa, **meta)
+
+    @classmethod
+    def load(cls, path: str) -> "Net":
+        with np.load(path, allow_pickle=False) as z:
+            net = cls(z["params"], z["in_scale"], z["out_scale"])
+            if "kappa" in z:
+                net.kappa = float(z["kappa"])
+        return net
 
 
 BATCHED, LOOPED, GRAPH = 0, 1, 2

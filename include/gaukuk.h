@@ -132,6 +132,65 @@ gk_status gk_ens_time(gk_ens* ens, int nsteps, int mode, double* ms);
 gk_status gk_ens_get_time(gk_ens* ens, int p, double* t, double* dt, int* step);
 const char* gk_ens_last_error(const gk_ens* ens);
 
+/* ---- Learned flux correction (no counterpart in the reference, whose coarse flux is plain HLLC,
+ * src/flux/hllc.cpp:22-103).  A network (gk_net) is an MLP 16 -> 64 -> 64 -> 4 with ReLU, run on
+ * BF16 matrix cores with FP32 accumulation, plus its Adam state and a device-resident data set.
+ * Parameters: 5508 floats, W1[64][16], b1[64], W2[64][64], b2[64], W3[4][64], b3[4] (row-major,
+ * W[out][in]).  Network input = feature * in_scale; flux correction = output * out_scale, in the
+ * units described in gk_learned.hip (tangential face frame over rho c^k of the face means). */
+#define GK_NET_PARAMS 5508
+typedef struct gk_net gk_net;
+gk_status gk_net_create(gk_net** out);
+void gk_net_destroy(gk_net* net);
+/* Setting parameters also clears the Adam moments and the minibatch draw counter. */
+gk_status gk_net_set_params(gk_net* net, const float* params);
+gk_status gk_net_get_params(gk_net* net, float* params);
+gk_status gk_net_set_scales(gk_net* net, const float in_scale[16], const float out_scale[4]);
+gk_status gk_net_get_scales(gk_net* net, float in_scale[16], float out_scale[4]);
+/* Dissipation bound (default 0.5): each correction component, and each collected training target,
+ * is clipped to +-kappa * 1/2 S |U_R - U_L| of the PLM face states (kappa = 0: no correction). */
+gk_status gk_net_set_kappa(gk_net* net, float kappa);
+gk_status gk_net_get_kappa(gk_net* net, float* kappa);
+/* y[k][0..3] = network output (before out_scale) for x[k][0..15] (raw features); host buffers. */
+gk_status gk_net_forward(gk_net* net, long long count, const float* x, float* y);
+/* Gradient of L = sum_k wt_k sum_o (y_ko - t_ko / out_scale_o)^2 / (4 count); host buffers. */
+gk_status gk_net_gradient(gk_net* net, long long count, const float* x, const float* t, const float* wt,
+                          float* grad, double* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected) with a host gradient. */
+gk_status gk_net_adam(gk_net* net, const float* grad, float lr);
+/* Data set on the device: reserve room, append host samples, read back a range (x, t, wt may be
+ * NULL) and its size, weighted mean squares of inputs and targets. */
+gk_status gk_net_reserve(gk_net* net, long long capacity);
+gk_status gk_net_add_samples(gk_net* net, long long count, const float* x, const float* t, const float* wt);
+gk_status gk_net_dataset(gk_net* net, long long start, long long count, float* x, float* t, float* wt,
+                         long long* size);
+gk_status gk_net_dataset_stats(gk_net* net, double ms_x[16], double ms_t[4], double* weight);
+/* iters Adam steps on minibatches of blocks * 64 * tiles_per_wave samples drawn (with replacement,
+ * from `seed`) from the data set; loss_log (may be NULL) receives each step's loss. */
+gk_status gk_net_train(gk_net* net, int iters, int blocks, int tiles_per_wave, float lr, unsigned long long seed,
+                       float* loss_log);
+const char* gk_net_last_error(const gk_net* net);
+
+/* Training data from a fine FP64 run (both grids FP64, same integrator and gamma, fine = f x coarse
+ * in both axes): restrict the fine state onto the coarse grid, take the coarse grid's features at
+ * every face, run one recorded coarse step (length D) and the fine run over exactly D, recorded;
+ * append one sample per coarse face (target: restricted fine flux minus coarse flux, weight 0
+ * where the shock sensor fires).  The fine run keeps going from t + D. */
+gk_status gk_net_collect(gk_net* net, gk_ctx* fine, gk_ctx* coarse, double* dt_coarse, long long* added);
+/* Flux recording: while on, every stage adds (stage weight) * dt * flux at every face: mass, m_x,
+ * m_y, E planes of (nx+1)*ny x faces then nx*(ny+1) y faces (face i between cells i-1 and i).
+ * Turning it on zeroes the sums. */
+gk_status gk_flux_record(gk_ctx* ctx, int on);
+gk_status gk_flux_download(gk_ctx* ctx, double* acc);
+/* Conservative average of the fine grid's state onto the coarse grid (the coarse grid takes the
+ * fine grid's time and restarts its step count). */
+gk_status gk_restrict(gk_ctx* fine, gk_ctx* coarse);
+/* Attach a network to a precision-32 PLM grid without gravity (NULL detaches): its stages then
+ * run the corrected kernel.  The network must outlive the attachment. */
+gk_status gk_set_net(gk_ctx* ctx, gk_net* net);
+/* Face counts since the last reset: corrected, shock-sensor fallbacks, positivity fallbacks. */
+gk_status gk_net_stats(gk_ctx* ctx, long long out[3], int reset);
+
 /* Text of the last error on this context (or of gk_create when ctx is NULL). */
 const char* gk_last_error(const gk_ctx* ctx);
 /* Library build identification (gfx target, policies). */
