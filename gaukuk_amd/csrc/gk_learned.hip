@@ -421,22 +421,33 @@ __global__ __launch_bounds__(256) void restrict_kernel(const RF* fine, long long
     }
 }
 
+// Reciprocal and root for the learned path's own arithmetic (features, bound): the hardware
+// approximations in FP32 (v_rcp_f32, v_sqrt_f32, within 1 ulp), exact operators in FP64.
+__device__ __forceinline__ float lrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double lrcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float lsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double lsqrt(double x) { return __builtin_sqrt(x); }
+
 // Face means, features, sensor: shared by the data pipeline and the corrected stage kernel.
 // st[k] = (rho, v_n, v_t, p) of cell k = 0..3 across the face.
 template <class R>
 struct FaceFrame {
     R rb, pb, cb, vtb;
+    R irb, ipb, icb;   // reciprocals of the means
     __device__ __forceinline__ void init(const R st[4][4], R gamma) {
         rb = R(0.5) * (st[1][0] + st[2][0]);
         pb = R(0.5) * (st[1][3] + st[2][3]);
-        cb = gsqrt(gamma * pb / rb);
+        irb = lrcp(rb);
+        ipb = lrcp(pb);
+        cb = lsqrt(gamma * pb * irb);
+        icb = lrcp(cb);
         vtb = R(0.5) * (st[1][2] + st[2][2]);
     }
     __device__ __forceinline__ void features(const R c[4], R f[4]) const {
-        f[0] = c[0] / rb - R(1);
-        f[1] = c[1] / cb;
-        f[2] = (c[2] - vtb) / cb;
-        f[3] = c[3] / pb - R(1);
+        f[0] = c[0] * irb - R(1);
+        f[1] = c[1] * icb;
+        f[2] = (c[2] - vtb) * icb;
+        f[3] = c[3] * ipb - R(1);
     }
     // network output (times out_scale) -> lab-frame (mass, m_n, m_t, E) correction
     __device__ __forceinline__ void to_lab(const R y[4], R d[4]) const {
@@ -454,15 +465,13 @@ struct FaceFrame {
         y[3] = (d[3] - vtb * d[2] + R(0.5) * vtb * vtb * d[0]) / (rb * cb * cb * cb);
     }
 };
+// max_k |p_k+1 - p_k| / min(p_k, p_k+1) > 0.1, written without the quotient (p > 0)
 template <class R>
 __device__ __forceinline__ bool sensor_fires(const R p[4]) {
-    R m = R(0);
+    bool fire = false;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const R d = p[k + 1] - p[k];
-        m = smax(m, gabs(d) / smin(p[k], p[k + 1]));
-    }
-    return m > R(0.1);
+    for (int k = 0; k < 3; k++) fire |= gabs(p[k + 1] - p[k]) > R(0.1) * smin(p[k], p[k + 1]);
+    return fire;
 }
 
 // The dissipation bound D_q in the units of the network's output (face frame moving with vt_bar,
@@ -475,12 +484,13 @@ __device__ __forceinline__ bool sensor_fires(const R p[4]) {
 template <class R>
 __device__ __forceinline__ void dissipation_bound(const R f[16], R gamma, R D[4]) {
     R w[4][4];
+    const R ig = lrcp(gamma);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         w[k][0] = R(1) + f[4 * k];
         w[k][1] = f[4 * k + 1];
         w[k][2] = f[4 * k + 2];
-        w[k][3] = (R(1) + f[4 * k + 3]) / gamma;
+        w[k][3] = (R(1) + f[4 * k + 3]) * ig;
     }
     R l[4], r[4];
 #pragma unroll
@@ -490,11 +500,11 @@ __device__ __forceinline__ void dissipation_bound(const R f[16], R gamma, R D[4]
     }
     const R rho = R(0.5) * (l[0] + r[0]), vn = R(0.5) * (l[1] + r[1]);
     const R vt = R(0.5) * (l[2] + r[2]), p = R(0.5) * (l[3] + r[3]);
-    const R c = gsqrt(gamma * p / rho), c2 = c * c;
+    const R c2 = gamma * p * lrcp(rho), c = lsqrt(c2), ic2 = lrcp(c2);
     const R dr = r[0] - l[0], dvn = r[1] - l[1], dvt = r[2] - l[2], dp = r[3] - l[3];
-    const R a1 = (dp - rho * c * dvn) / (R(2) * c2), a4 = (dp + rho * c * dvn) / (R(2) * c2);
-    const R a2 = dr - dp / c2, a3 = rho * dvt;
-    const R H = c2 / (gamma - R(1)) + R(0.5) * (vn * vn + vt * vt);
+    const R a1 = R(0.5) * (dp - rho * c * dvn) * ic2, a4 = R(0.5) * (dp + rho * c * dvn) * ic2;
+    const R a2 = dr - dp * ic2, a3 = rho * dvt;
+    const R H = c2 * lrcp(gamma - R(1)) + R(0.5) * (vn * vn + vt * vt);
     const R w1 = gabs(vn - c) * gabs(a1), w2 = gabs(vn) * gabs(a2);
     const R w3 = gabs(vn) * gabs(a3), w4 = gabs(vn + c) * gabs(a4);
     D[0] = R(0.5) * (w1 + w2 + w4);
@@ -600,10 +610,18 @@ __global__ __launch_bounds__(256) void face_targets_kernel(const double* facc, i
 //   2. PLM + HLLC at the tile's 272 x faces and 272 y faces -> LDS
 //   3. the MLP on 34 tiles of 16 faces (BF16 MFMA), sensor and positivity tests, correction
 //   4. flux divergence, RK combine, floors, store, max wave speed
+// Weights image: BF16 matrices and FP32 biases/scales in the LDS layout, rebuilt when the
+// network's parameters or scales change.  kImageCopies replicas (one per block of this kernel):
+// every workgroup of a stage reads the image at the same moment, and one copy read by all of
+// them queues on the few L2 channels that hold it.
+constexpr int kImageCopies = 32;
+__global__ __launch_bounds__(256) void mlp_image_kernel(const float* params, const float* scales, mlp::Smem* img) {
+    mlp::stage(img[blockIdx.x], params, scales, scales + mlp::kIn, threadIdx.x, 256);
+}
+
 struct NetArgs {
-    const float* params;
-    const float* scales;              // in_scale[16], out_scale[4]
-    unsigned long long* stats;        // corrected, sensor fallbacks, positivity fallbacks
+    const mlp::Smem* image;           // prebuilt LDS images of the weights (kImageCopies replicas)
+    unsigned long long* stats;        // per tile: corrected, sensor fallbacks, positivity fallbacks
     float kappa;                      // dissipation bound factor
     int enabled;
 };
@@ -613,14 +631,22 @@ constexpr int kLE = kLT + 2 * kHalo;       // tile edge with halo
 constexpr int kLFx = (kLT + 1) * kLT;      // x faces per tile (y faces: the same count)
 constexpr int kLFaces = 2 * kLFx;          // 544 = 34 tiles of 16
 
+constexpr int kLThreads = 512;             // 8 waves per tile: two per SIMD
+constexpr int kLWaves = kLThreads / kWave;
+
 template <bool THREE, bool LAST>
-__global__ __launch_bounds__(256) void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
+__global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
     DevClock* clk = a.clock;
     if (!clk->active) return;
     __shared__ mlp::Smem s;
     __shared__ float sw[5][kLE][kLE + 1];
     __shared__ float su[5][kLE][kLE + 1];
     __shared__ float sf[5][kLFaces];
+    __shared__ unsigned long long scount[kLWaves][3];
+    __shared__ unsigned short sx[kLFaces * mlp::kP16];   // scaled BF16 network inputs per face
+    __shared__ float smeta[kLFaces][8];                  // bound D[4], rho_bar, c_bar, vt_bar, sensor
+    __shared__ float sy[kLFaces][4];                     // network outputs
+    __shared__ double swmax[kLWaves];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tiles_x = (a.nx + kLT - 1) / kLT;
     const int i0 = (blockIdx.x % tiles_x) * kLT, j0 = (blockIdx.x / tiles_x) * kLT;
@@ -630,16 +656,40 @@ __global__ __launch_bounds__(256) void learned_stage_kernel(const StageArgs<floa
     float* out = a.out;
     if (a.pingpong && clk->cur) { in = a.out; out = const_cast<float*>(a.in); }
     const float gamma = a.eos.gamma;
-    if (net.enabled) mlp::stage(s, net.params, net.scales, net.scales + mlp::kIn, tid, 256);
+    // Every global load of the kernel is issued here, before any result is used, so the
+    // workgroup waits out one memory latency: the tile and halo cells (400, one per thread),
+    // the combine operand of this thread's cell, and the weights image.
+    constexpr int kCells = kLE * kLE;
+    static_assert(kCells <= kLThreads, "one cell per thread");
+    const int kc = min(tid, kCells - 1);
+    const int ly = kc / kLE, lx = kc - ly * kLE;
+    int kx, ky;
+    const int si = bc_remap(min(max(i0 + lx - kHalo, -2), a.nx + 1), a.nx, a.bc[0], a.bc[1], kx);
+    const int sj = bc_remap(min(max(j0 + ly - kHalo, -2), a.ny + 1), a.ny, a.bc[2], a.bc[3], ky);
+    const long long oc = (long long)sj * a.pitch + si;
+    float u[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) u[n] = in[oc + n * a.plane];
+    const int cx = tid % kLT, cy = min(tid / kLT, kLT - 1);
+    const int i = i0 + cx, j = j0 + cy;
+    const bool mine = tid < kLT * kLT && i < a.nx && j < a.ny;
+    const long long o = (long long)min(j, a.ny - 1) * a.pitch + min(i, a.nx - 1);
+    float bse[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) bse[n] = THREE ? a.base[o + n * a.plane] : 0.f;
+    mlp::ImageCopy<kLThreads> wimg;
+    if (net.enabled) wimg.load(net.image + blockIdx.x % kImageCopies, tid);
+    __builtin_amdgcn_sched_barrier(0);   // keep the loads above from being split by waits
 
-    for (int k = tid; k < kLE * kLE; k += 256) {
-        const int ly = k / kLE, lx = k - ly * kLE;
-        float u[5], w[5];
-        load_cell(in, a.plane, a.pitch, a.nx, a.ny, a.bc, i0 + lx - kHalo, j0 + ly - kHalo, u);
+    bc_transform(u, 1, kx);   // x rule, then y rule (corner cells are never read)
+    bc_transform(u, 2, ky);
+    if (tid < kCells) {
+        float w[5];
         a.eos.cons_to_prim(u, w);
 #pragma unroll
         for (int n = 0; n < 5; n++) { su[n][ly][lx] = u[n]; sw[n][ly][lx] = w[n]; }
     }
+    if (net.enabled) wimg.store(s, tid);
     __syncthreads();
 
     // face f: x faces fx = 0..16 of row fy (between cells fx-1 and fx), then y faces
@@ -647,7 +697,7 @@ __global__ __launch_bounds__(256) void learned_stage_kernel(const StageArgs<floa
         if (f < kLFx) { const int fy = f / (kLT + 1), fx = f - fy * (kLT + 1); cy = fy + kHalo; cx = fx + k; }
         else { const int f2 = f - kLFx, fy = f2 / kLT, fx = f2 - fy * kLT; cy = fy + k; cx = fx + kHalo; }
     };
-    for (int f = tid; f < kLFaces; f += 256) {
+    for (int f = tid; f < kLFaces; f += kLThreads) {
         float w[4][5], F[5];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -663,15 +713,8 @@ __global__ __launch_bounds__(256) void learned_stage_kernel(const StageArgs<floa
     __syncthreads();
 
     if (net.enabled) {
-        mlp::Regs wr;
-        mlp::load_regs(wr, s, lane);
-        const int g = lane >> 4, r = lane & 15;
-        float insc[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) insc[i] = s.in_scale[4 * g + i];
-        unsigned long long n_corr = 0, n_sensor = 0, n_pos = 0;
-        for (int q = wv; q * 16 < kLFaces; q += 4) {
-            const int f = q * 16 + r;
+        // M1, one face per thread: frame, features, dissipation bound, sensor
+        for (int f = tid; f < kLFaces; f += kLThreads) {
             const bool yface = f >= kLFx;
             float st[4][4];
 #pragma unroll
@@ -685,85 +728,114 @@ __global__ __launch_bounds__(256) void learned_stage_kernel(const StageArgs<floa
             }
             FaceFrame<float> fr;
             fr.init(st, gamma);
-            float fe[4];
-            fr.features(st[g], fe);
-            mlp::f32x4 xs = {fe[0] * insc[0], fe[1] * insc[1], fe[2] * insc[2], fe[3] * insc[3]};
-            const mlp::f32x4 y = mlp::forward_tile(wr, mlp::pack4(xs));
-            if (g == 0) {
-                float fa[16], bd[4];
+            float fa[16], bd[4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) fr.features(st[k], fa + 4 * k);
-                dissipation_bound(fa, gamma, bd);
-                float yh[4];
+            for (int k = 0; k < 4; k++) fr.features(st[k], fa + 4 * k);
+            dissipation_bound(fa, gamma, bd);
+            const float p[4] = {st[0][3], st[1][3], st[2][3], st[3][3]};
 #pragma unroll
-                for (int q = 0; q < 4; q++) yh[q] = clip(y[q] * s.out_scale[q], net.kappa * bd[q]);
-                float d[4];
-                fr.to_lab(yh, d);
-                float d5[5];
-                d5[0] = d[0];
-                d5[1] = yface ? d[2] : d[1];
-                d5[2] = yface ? d[1] : d[2];
-                d5[3] = 0.f;
-                d5[4] = d[3];
-                const float p[4] = {st[0][3], st[1][3], st[2][3], st[3][3]};
-                const bool fired = sensor_fires(p);
-                // one-face shares of the two neighbours (cells 1 and 2)
-                const float nu4 = 4.f * (yface ? dtdy : dtdx);
-                int ly, lx, hy, hx;
-                face_cell(f, 1, ly, lx);
-                face_cell(f, 2, hy, hx);
-                float uL[5], uR[5];
-#pragma unroll
-                for (int n = 0; n < 5; n++) {
-                    uL[n] = su[n][ly][lx] - nu4 * d5[n];
-                    uR[n] = su[n][hy][hx] + nu4 * d5[n];
-                }
-                const float pL = (gamma - 1.f) * (uL[4] - 0.5f * (uL[1] * uL[1] + uL[2] * uL[2] + uL[3] * uL[3]) / uL[0]);
-                const float pR = (gamma - 1.f) * (uR[4] - 0.5f * (uR[1] * uR[1] + uR[2] * uR[2] + uR[3] * uR[3]) / uR[0]);
-                bool ok = uL[0] > 0.f && uR[0] > 0.f && pL > 0.f && pR > 0.f;
-#pragma unroll
-                for (int n = 0; n < 5; n++) ok = ok && __builtin_isfinite(d5[n]);
-                if (!fired && ok) {
-#pragma unroll
-                    for (int n = 0; n < 5; n++) sf[n][f] += d5[n];
-                }
-                // counts: each grid face once (faces on a tile's high edge belong to the next tile)
-                const int fl = yface ? (f - kLFx) / kLT : f % (kLT + 1);
-                const bool own = fl < kLT && (yface ? j0 + fl < a.ny : i0 + fl < a.nx);
-                if (own) {
-                    if (fired) n_sensor++;
-                    else if (ok) n_corr++;
-                    else n_pos++;
-                }
+            for (int q = 0; q < 4; q++) {
+                const mlp::f32x4 v = {fa[4 * q] * s.in_scale[4 * q], fa[4 * q + 1] * s.in_scale[4 * q + 1],
+                                      fa[4 * q + 2] * s.in_scale[4 * q + 2], fa[4 * q + 3] * s.in_scale[4 * q + 3]};
+                *reinterpret_cast<mlp::bf16x4*>(&sx[f * mlp::kP16 + 4 * q]) = mlp::pack4(v);
             }
+            *reinterpret_cast<float4*>(&smeta[f][0]) = make_float4(bd[0], bd[1], bd[2], bd[3]);
+            *reinterpret_cast<float4*>(&smeta[f][4]) = make_float4(fr.rb, fr.cb, fr.vtb, sensor_fires(p) ? 1.f : 0.f);
         }
-        if (net.stats) {
-            for (int m = 1; m < 64; m <<= 1) {
-                n_corr += __shfl_xor(n_corr, m);
-                n_sensor += __shfl_xor(n_sensor, m);
-                n_pos += __shfl_xor(n_pos, m);
-            }
-            if (lane == 0) {
-                atomicAdd(net.stats + 0, n_corr);
-                atomicAdd(net.stats + 1, n_sensor);
-                atomicAdd(net.stats + 2, n_pos);
+        __syncthreads();
+        // M2, per wave: the network on tiles of 16 faces (lane (g, s) feeds inputs 4g..4g+3 of face s)
+        {
+            mlp::Regs wr;
+            mlp::load_regs(wr, s, lane);
+            const int g = lane >> 4, r = lane & 15;
+            for (int q = wv; q * 16 < kLFaces; q += kLWaves) {
+                const int f = q * 16 + r;
+                const mlp::f32x4 y = mlp::forward_tile(wr, mlp::lds4(&sx[f * mlp::kP16 + 4 * g]));
+                if (g == 0) *reinterpret_cast<float4*>(&sy[f][0]) = make_float4(y[0], y[1], y[2], y[3]);
             }
         }
         __syncthreads();
+        // M3, one face per thread: bound, lab frame, fallbacks, corrected flux
+        unsigned long long n_corr = 0, n_sensor = 0, n_pos = 0;
+        for (int f = tid; f < kLFaces; f += kLThreads) {
+            const bool yface = f >= kLFx;
+            const float4 bd = *reinterpret_cast<const float4*>(&smeta[f][0]);
+            const float4 mt = *reinterpret_cast<const float4*>(&smeta[f][4]);
+            const float4 yv = *reinterpret_cast<const float4*>(&sy[f][0]);
+            FaceFrame<float> fr;
+            fr.rb = mt.x;
+            fr.cb = mt.y;
+            fr.vtb = mt.z;
+            const float yh[4] = {clip(yv.x * s.out_scale[0], net.kappa * bd.x), clip(yv.y * s.out_scale[1], net.kappa * bd.y),
+                                 clip(yv.z * s.out_scale[2], net.kappa * bd.z), clip(yv.w * s.out_scale[3], net.kappa * bd.w)};
+            float d[4];
+            fr.to_lab(yh, d);
+            float d5[5];
+            d5[0] = d[0];
+            d5[1] = yface ? d[2] : d[1];
+            d5[2] = yface ? d[1] : d[2];
+            d5[3] = 0.f;
+            d5[4] = d[3];
+            const bool fired = mt.w != 0.f;
+            // one-face shares of the two neighbours (cells 1 and 2)
+            const float nu4 = 4.f * (yface ? dtdy : dtdx);
+            int ly, lx, hy, hx;
+            face_cell(f, 1, ly, lx);
+            face_cell(f, 2, hy, hx);
+            float uL[5], uR[5];
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+                uL[n] = su[n][ly][lx] - nu4 * d5[n];
+                uR[n] = su[n][hy][hx] + nu4 * d5[n];
+            }
+            // p > 0 <=> rho E - |m|^2 / 2 > 0 for rho > 0
+            const float qL = uL[0] * uL[4] - 0.5f * (uL[1] * uL[1] + uL[2] * uL[2] + uL[3] * uL[3]);
+            const float qR = uR[0] * uR[4] - 0.5f * (uR[1] * uR[1] + uR[2] * uR[2] + uR[3] * uR[3]);
+            bool ok = uL[0] > 0.f && uR[0] > 0.f && qL > 0.f && qR > 0.f;
+#pragma unroll
+            for (int n = 0; n < 5; n++) ok = ok && __builtin_isfinite(d5[n]);
+            if (!fired && ok) {
+#pragma unroll
+                for (int n = 0; n < 5; n++) sf[n][f] += d5[n];
+            }
+            // counts: each grid face once (faces on a tile's high edge belong to the next tile)
+            const int fl = yface ? (f - kLFx) / kLT : f % (kLT + 1);
+            const bool own = fl < kLT && (yface ? j0 + fl < a.ny : i0 + fl < a.nx);
+            if (own) {
+                if (fired) n_sensor++;
+                else if (ok) n_corr++;
+                else n_pos++;
+            }
+        }
+        // face counts: wave sums to LDS; after the barrier one thread adds them to this tile's
+        // slot (plain read-modify-write: a tile belongs to one workgroup, launches are ordered)
+        for (int m = 1; m < 64; m <<= 1) {
+            n_corr += __shfl_xor(n_corr, m);
+            n_sensor += __shfl_xor(n_sensor, m);
+            n_pos += __shfl_xor(n_pos, m);
+        }
+        if (lane == 0) { scount[wv][0] = n_corr; scount[wv][1] = n_sensor; scount[wv][2] = n_pos; }
+        __syncthreads();
+        if (net.stats && tid == 0) {
+            unsigned long long* slot = net.stats + 3 * (long long)blockIdx.x;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                unsigned long long t = 0;
+                for (int w = 0; w < kLWaves; w++) t += scount[w][k];
+                slot[k] += t;
+            }
+        }
     }
 
-    const int cx = tid % kLT, cy = tid / kLT;
-    const int i = i0 + cx, j = j0 + cy;
     float cacc = 0.f;
-    if (i < a.nx && j < a.ny) {
+    if (mine) {
         const int fxl = cy * (kLT + 1) + cx, fyl = kLFx + cy * kLT + cx;
-        const long long o = (long long)j * a.pitch + i;
         float res[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
             const float dF = (sf[n][fxl + 1] - sf[n][fxl]) * dtdx + (sf[n][fyl + kLT] - sf[n][fyl]) * dtdy;
             const float ut = su[n][cy + kHalo][cx + kHalo];
-            if (THREE) res[n] = a.c1 * a.base[o + n * a.plane] + a.c2 * ut - a.c3 * dF;
+            if (THREE) res[n] = a.c1 * bse[n] + a.c2 * ut - a.c3 * dF;
             else res[n] = a.c1 * ut - a.c2 * dF;
         }
         a.eos.floors(res);
@@ -772,8 +844,15 @@ __global__ __launch_bounds__(256) void learned_stage_kernel(const StageArgs<floa
         for (int n = 0; n < 5; n++) out[o + n * a.plane] = res[n];
     }
     if (LAST) {
+        // one atomic per workgroup: simultaneous atomics on one address serialise
         const double m = wave_max((double)cacc);
-        if (lane == 0) atomicMax(&clk->cmax_bits[clk->parity ^ 1], dbits(m));
+        if (lane == 0) swmax[wv] = m;
+        __syncthreads();
+        if (tid == 0) {
+            double mm = swmax[0];
+            for (int w = 1; w < kLWaves; w++) mm = smax(mm, swmax[w]);
+            atomicMax(&clk->cmax_bits[clk->parity ^ 1], dbits(mm));
+        }
     }
 }
 
@@ -802,6 +881,9 @@ struct gk_net {
     long long stage_cap = 0;
     double* stats_partial = nullptr;
     float kappa = 0.5f;        // dissipation bound of corrections and training targets
+    mlp::Smem* image = nullptr;  // LDS image of the weights for the corrected stage kernel
+    long long version = 0;       // bumped whenever parameters or scales change
+    long long image_version = -1;
     hipStream_t stream = nullptr;
     std::string err;
 };
@@ -850,6 +932,7 @@ gk_status net_step(gk_net* n, const float* X, const float* T, const float* Wt, l
     a.inv_norm = (float)(1.0 / (4.0 * (double)(sequential ? count : batch)));
     a.partial = n->partial; a.loss_partial = n->loss_partial;
     hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(256), 0, n->stream, a);
+    if (apply) n->version++;
     float bc1 = 1.f, bc2 = 1.f;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     if (apply) {
@@ -904,17 +987,25 @@ static void record_launch(gk_ctx* c, size_t q) {
                            EosPolicy<double>::make(c->phys.gamma), c->clock, c->rec_w[q], c->acc);
 }
 
+static int learned_tiles(const gk_ctx* c) { return ((c->nx + kLT - 1) / kLT) * ((c->ny + kLT - 1) / kLT); }
+
 static bool learned_launch(gk_ctx* c, const StageSpec& s) {
     if (!c->net || c->prec != 32) return false;
     StageArgs<float> a = make_args<float>(c, s);
+    gk_net* net = c->net;
+    if (net->image_version != net->version) {
+        // the network's own stream has finished its updates (every net call that changes
+        // parameters synchronises it); the image is rebuilt in this grid's stream order
+        hipLaunchKernelGGL(mlp_image_kernel, dim3(kImageCopies), dim3(256), 0, c->stream, net->params, net->scales,
+                           net->image);
+        net->image_version = net->version;
+    }
     NetArgs na;
-    na.params = c->net->params;
-    na.scales = c->net->scales;
+    na.image = net->image;
     na.stats = c->lstats;
     na.kappa = c->net->kappa;
     na.enabled = 1;
-    const int tiles = ((c->nx + kLT - 1) / kLT) * ((c->ny + kLT - 1) / kLT);
-    dim3 grid(tiles), block(256);
+    dim3 grid(learned_tiles(c)), block(kLThreads);
     if (s.three) {
         if (s.last) hipLaunchKernelGGL((learned_stage_kernel<true, true>), grid, block, 0, c->stream, a, na);
         else hipLaunchKernelGGL((learned_stage_kernel<true, false>), grid, block, 0, c->stream, a, na);
@@ -939,7 +1030,8 @@ gk_status gk_net_create(gk_net** out) {
         hipMalloc(&n->grad, sizeof(float) * mlp::kParams) != hipSuccess ||
         hipMalloc(&n->scales, sizeof(float) * (mlp::kIn + mlp::kOut)) != hipSuccess ||
         hipMalloc(&n->loss, sizeof(float)) != hipSuccess ||
-        hipMalloc(&n->stats_partial, sizeof(double) * 21 * 1024) != hipSuccess)
+        hipMalloc(&n->stats_partial, sizeof(double) * 21 * 1024) != hipSuccess ||
+        hipMalloc(&n->image, sizeof(mlp::Smem) * kImageCopies) != hipSuccess)
         return bail(GK_ENOMEM);
     std::vector<float> zero(mlp::kParams, 0.f), ones(mlp::kIn + mlp::kOut, 1.f);
     if (hipMemcpy(n->params, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice) != hipSuccess ||
@@ -955,7 +1047,7 @@ void gk_net_destroy(gk_net* n) {
     if (!n) return;
     if (n->stream) (void)hipStreamSynchronize(n->stream);
     void* bufs[] = {n->params, n->m, n->v, n->scales, n->grad, n->loss, n->partial, n->loss_partial,
-                    n->X, n->T, n->Wt, n->feat, n->meta, n->stats_partial};
+                    n->X, n->T, n->Wt, n->feat, n->meta, n->stats_partial, n->image};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (n->stream) (void)hipStreamDestroy(n->stream);
@@ -971,6 +1063,7 @@ gk_status gk_net_set_params(gk_net* n, const float* params) {
     GK_NET_HIP(n, hipMemcpy(n->v, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice));
     n->adam_t = 0;
     n->train_iter = 0;
+    n->version++;
     return GK_OK;
 }
 
@@ -989,6 +1082,7 @@ gk_status gk_net_set_scales(gk_net* n, const float in_scale[16], const float out
     std::memcpy(h, in_scale, sizeof(float) * mlp::kIn);
     std::memcpy(h + mlp::kIn, out_scale, sizeof(float) * mlp::kOut);
     GK_NET_HIP(n, hipMemcpy(n->scales, h, sizeof h, hipMemcpyHostToDevice));
+    n->version++;
     return GK_OK;
 }
 
@@ -1061,6 +1155,7 @@ gk_status gk_net_adam(gk_net* n, const float* grad, float lr) {
     if (st != GK_OK) return st;
     GK_NET_HIP(n, hipMemcpy(n->partial, grad, sizeof(float) * mlp::kParams, hipMemcpyHostToDevice));
     n->adam_t++;
+    n->version++;
     const float bc1 = (float)(1.0 - std::pow(0.9, (double)n->adam_t));
     const float bc2 = (float)(1.0 - std::pow(0.999, (double)n->adam_t));
     hipLaunchKernelGGL(adam_kernel, dim3((mlp::kParams + 255) / 256), dim3(256), 0, n->stream, n->params, n->m, n->v,
@@ -1292,8 +1387,9 @@ gk_status gk_set_net(gk_ctx* c, gk_net* n) {
     if (c->phys.rc_order == 1) return fail(c, GK_EINVAL, "the corrected stage uses PLM (RcOrder 2)");
     if (c->nx < kLT || c->ny < kLT) return fail(c, GK_EINVAL, "the corrected stage needs NX, NY >= 16");
     if (!c->lstats) {
-        GK_TRY_HIP(c, hipMalloc(&c->lstats, sizeof(unsigned long long) * 3));
-        GK_TRY_HIP(c, hipMemset(c->lstats, 0, sizeof(unsigned long long) * 3));
+        const size_t bytes = sizeof(unsigned long long) * 3 * (size_t)learned_tiles(c);
+        GK_TRY_HIP(c, hipMalloc(&c->lstats, bytes));
+        GK_TRY_HIP(c, hipMemset(c->lstats, 0, bytes));
     }
     c->net = n;
     return GK_OK;
@@ -1301,13 +1397,17 @@ gk_status gk_set_net(gk_ctx* c, gk_net* n) {
 
 gk_status gk_net_stats(gk_ctx* c, long long out[3], int reset) {
     if (!c || !out) return GK_EINVAL;
-    unsigned long long h[3] = {0, 0, 0};
+    long long sum[3] = {0, 0, 0};
     if (c->lstats) {
+        const int tiles = learned_tiles(c);
+        std::vector<unsigned long long> h(3 * (size_t)tiles);
         GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
-        GK_TRY_HIP(c, hipMemcpy(h, c->lstats, sizeof h, hipMemcpyDeviceToHost));
-        if (reset) GK_TRY_HIP(c, hipMemset(c->lstats, 0, sizeof h));
+        GK_TRY_HIP(c, hipMemcpy(h.data(), c->lstats, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+        if (reset) GK_TRY_HIP(c, hipMemset(c->lstats, 0, sizeof(unsigned long long) * h.size()));
+        for (int t = 0; t < tiles; t++)
+            for (int k = 0; k < 3; k++) sum[k] += (long long)h[3 * (size_t)t + k];
     }
-    for (int k = 0; k < 3; k++) out[k] = (long long)h[k];
+    for (int k = 0; k < 3; k++) out[k] = sum[k];
     return GK_OK;
 }
 

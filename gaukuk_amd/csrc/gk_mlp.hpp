@@ -57,12 +57,46 @@ __device__ __forceinline__ bf16x4 lds4(const unsigned short* p) {
 
 // Network weights in LDS: BF16 matrices (padded rows), FP32 biases and the fixed input/output
 // scales (network input = feature * in_scale; correction = output * out_scale).
-struct Smem {
+// The same struct in device memory is the prebuilt image a kernel copies into LDS.
+struct alignas(16) Smem {
     unsigned short w1[kHid * kP16];
     unsigned short w2[kHid * kP64];
     unsigned short w3[kOut * kP64];
     float b1[kHid], b2[kHid], b3[kOut];
     float in_scale[kIn], out_scale[kOut];
+};
+static_assert(sizeof(Smem) % 16 == 0, "the image is copied in 16-byte pieces");
+constexpr int kImageVec = (int)(sizeof(Smem) / 16);
+
+// A prebuilt image is copied into LDS in 16-byte pieces, NPER per thread of an NT-thread block:
+// load_image issues the loads, store_image (later, so other work hides the latency) writes LDS.
+template <int NT>
+struct ImageCopy {
+    static constexpr int kPer = (kImageVec + NT - 1) / NT;
+    uint4 v[kPer];
+    // Unconditional (clamped) loads: predicated ones are paired with their stores by the
+    // compiler and each waits out a full memory latency.  Four 4-byte loads per piece: measured
+    // on the corrected stage kernel at 256^2, 16-byte loads of the image took the load phase
+    // from 1.6 us to 8.5 us.
+    __device__ __forceinline__ void load(const Smem* img, int tid) {
+        const uint4* src = reinterpret_cast<const uint4*>(img);
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const unsigned* w = reinterpret_cast<const unsigned*>(src + min(tid + q * NT, kImageVec - 1));
+            v[q].x = w[0];
+            v[q].y = w[1];
+            v[q].z = w[2];
+            v[q].w = w[3];
+        }
+    }
+    __device__ __forceinline__ void store(Smem& s, int tid) const {
+        uint4* dst = reinterpret_cast<uint4*>(&s);
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const int k = tid + q * NT;
+            if (k < kImageVec) dst[k] = v[q];
+        }
+    }
 };
 // Backward operands: W2^T and W3^T (rows padded to 16 outputs with zeros).
 struct SmemT {
