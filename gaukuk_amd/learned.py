@@ -39,9 +39,16 @@ def restrict(u: np.ndarray, f: int) -> np.ndarray:
     return u.reshape(q, ny // f, f, nx // f, f).mean(axis=(2, 4))
 
 
+# The seeded perturbation is drawn on a 64^2 grid and held constant over its cells, so every grid
+# of 64^2 or finer starts from the same state and the fine run is a converged reference for the
+# coarse ones (noise drawn per fine cell cannot be represented by a coarse grid, and its chaotic
+# growth then dominates any coarse-vs-fine error).
+NOISE_CELLS = 64
+
+
 def kh(n: int, seed: int) -> problems.Problem:
-    """KH ICs on n^2, RK3, run by time (no step cap)."""
-    return problems.kelvin_helmholtz(n, seed=seed, integrator=3).with_(stepmax=-1)
+    """KH ICs on n^2, RK3, run by time (no step cap); seeded noise on the 64^2 grid."""
+    return problems.kelvin_helmholtz(n, seed=seed, integrator=3, noise_cells=NOISE_CELLS).with_(stepmax=-1)
 
 
 def init_params(seed: int) -> np.ndarray:
@@ -102,9 +109,10 @@ def rel_l1(a: np.ndarray, b: np.ndarray) -> list:
     return [float(np.sum(np.abs(a[q] - b[q])) / np.sum(np.abs(b[q]))) for q in (0, 1, 2, 4)]
 
 
-def timed_run(prob, precision: int, t_end: float, net=None, repeats: int = 2, stepmax: int = -1):
-    """Run prob to t_end (at most stepmax steps); returns (state, wall seconds of the fastest
-    repeat, steps, fallbacks, time reached)."""
+def timed_run(prob, precision: int, times, net=None, repeats: int = 2, stepmax: int = -1):
+    """Run prob through the output times (ascending; at most stepmax steps).  Returns the states
+    at those times, the wall seconds of advancing to the last one (fastest repeat, downloads
+    excluded), the step count, the fallback counts and the time reached."""
     from . import hip
     best = None
     out = None
@@ -113,43 +121,85 @@ def timed_run(prob, precision: int, t_end: float, net=None, repeats: int = 2, st
         if net is not None:
             s.set_net(net)
         s.synchronize()
-        t0 = time.perf_counter()
-        s.advance(t_end)
-        s.synchronize()
-        dt = time.perf_counter() - t0
-        if best is None or dt < best:
-            best = dt
-        out = (s.download(), s.step, s.net_stats() if net is not None else None, s.t)
+        wall = 0.0
+        states = []
+        for t in times:
+            t0 = time.perf_counter()
+            if s.time()[3]:
+                s.advance(t - s.t)
+            s.synchronize()
+            wall += time.perf_counter() - t0
+            states.append(s.download())
+        if best is None or wall < best:
+            best = wall
+        out = (states, s.step, s.net_stats() if net is not None else None, s.t)
         s.close()
     return out[0], best, out[1], out[2], out[3]
 
 
-def evaluate(net, seed: int = 1000, t_end: float = 0.5, n_fine: int = 1024, n_coarse: int = 256,
+def evaluate(net, seeds=(1000,), times=(0.3, 0.5), kappas=(0.5,), n_fine: int = 1024, n_coarse: int = 256,
              log=print) -> dict:
-    """L1 of each coarse solver against the restricted n_fine FP64 run, and wall times."""
+    """For held-out seeds: relative L1 (mass, m_x, m_y, E) at each output time of the plain FP64
+    solver at n_coarse and 2 n_coarse, the plain FP32 solver and the corrected FP32 solver (one per
+    kappa) at n_coarse, all restricted to n_coarse, against the n_fine FP64 run; and wall times."""
+    res = {"n_fine": n_fine, "n_coarse": n_coarse, "times": list(times), "seeds": {}}
+    for seed in seeds:
+        fp = kh(n_fine, seed)
+        truth, t_truth, st_truth, _, _ = timed_run(fp, 64, times, repeats=1)
+        refs = [restrict(u, n_fine // n_coarse) for u in truth]
+        r = {"fine": {"n": n_fine, "wall_s": t_truth, "steps": st_truth}}
+        cases = [("fp64", n_coarse, 64, None), ("fp64", 2 * n_coarse, 64, None), ("fp32", n_coarse, 32, None)]
+        cases += [(f"learned_fp32_k{k:g}", n_coarse, 32, k) for k in kappas]
+        plain_steps = None
+        for name, n, prec, kappa in cases:
+            prob = fp.with_(nx=n, ny=n, cons=restrict(fp.cons, n_fine // n))
+            if kappa is not None:
+                net.kappa = kappa
+            # a corrected run that loses stability would shrink dt without end: cap it
+            cap = -1 if kappa is None or plain_steps is None else 3 * plain_steps
+            us, wall, steps, stats, t = timed_run(prob, prec, times, net if kappa is not None else None, stepmax=cap)
+            if name == "fp64" and n == n_coarse:
+                plain_steps = steps
+            l1 = [rel_l1(restrict(u, n // n_coarse), ref) for u, ref in zip(us, refs)]
+            key = f"{name}_{n}"
+            r[key] = {"n": n, "precision": prec, "wall_s": wall, "steps": steps, "l1": l1,
+                      "reached": bool(abs(t - times[-1]) < 1e-9)}
+            if stats is not None:
+                r[key]["kappa"] = kappa
+                r[key]["faces"] = {"corrected": stats[0], "sensor": stats[1], "positivity": stats[2]}
+            log(f"seed {seed} {key}: wall {wall * 1e3:.1f} ms steps {steps} " +
+                " ".join(f"t={tt}: {np.round(x, 5).tolist()}" for tt, x in zip(times, l1)))
+        res["seeds"][str(seed)] = r
+    return res
+
+
+def one_step_error(net, seed: int = 1000, t0: float = 0.3, n_fine: int = 1024, n_coarse: int = 256) -> dict:
+    """Local (one coarse step) error of the plain and the corrected FP32 coarse solvers: from the
+    restricted state of the n_fine FP64 run at t0, one coarse step of length D (both solvers take
+    the same D, from the same state) against the restricted fine run over D.  Per variable (mass,
+    m_x, m_y, E): L1 of (coarse - restricted fine) over L1 of the true change."""
     from . import hip
     fp = kh(n_fine, seed)
-    truth, t_truth, st_truth, _, _ = timed_run(fp, 64, t_end, repeats=1)
-    res = {"seed": seed, "t_end": t_end, "fine": {"n": n_fine, "wall_s": t_truth, "steps": st_truth}}
-    ref = restrict(truth, n_fine // n_coarse)
-    cases = [("fp64", n_coarse, 64, None), ("fp64", 2 * n_coarse, 64, None),
-             ("fp32", n_coarse, 32, None), ("learned_fp32", n_coarse, 32, net)]
-    plain_steps = None
-    for name, n, prec, nn in cases:
-        prob = fp.with_(nx=n, ny=n, cons=restrict(fp.cons, n_fine // n))
-        # a corrected run that loses stability would shrink dt without end: cap it
-        cap = -1 if nn is None or plain_steps is None else 3 * plain_steps
-        u, wall, steps, stats, t = timed_run(prob, prec, t_end, nn, stepmax=cap)
-        if name == "fp64" and n == n_coarse:
-            plain_steps = steps
-        l1 = rel_l1(restrict(u, n // n_coarse), ref)
-        key = f"{name}_{n}"
-        res[key] = {"n": n, "precision": prec, "wall_s": wall, "steps": steps, "l1": l1,
-                    "reached_t_end": bool(abs(t - t_end) < 1e-9)}
-        if stats is not None:
-            res[key]["faces"] = {"corrected": stats[0], "sensor": stats[1], "positivity": stats[2]}
-        log(f"{key}: L1 {np.round(l1, 6).tolist()} wall {wall * 1e3:.1f} ms steps {steps} t {t:.4f}")
-    return res
+    f = n_fine // n_coarse
+    fine = hip.Solver(fp)
+    fine.advance(t0)
+    uf = fine.download()
+    u0 = restrict(uf, f)
+    cp = fp.with_(nx=n_coarse, ny=n_coarse, cons=u0)
+    plain = hip.Solver(cp, precision=32)
+    plain.run_steps(1)
+    up, D = plain.download(), plain.time()[1]
+    corr = hip.Solver(cp, precision=32)
+    corr.set_net(net)
+    corr.run_steps(1)
+    uc = corr.download()
+    fine.advance(D)
+    ref = restrict(fine.download(), f)
+
+    def err(u):
+        return [float(np.sum(np.abs(u[q] - ref[q])) / np.sum(np.abs(ref[q] - u0[q]))) for q in (0, 1, 2, 4)]
+
+    return {"seed": seed, "t0": t0, "dt": D, "plain": err(up), "corrected": err(uc)}
 
 
 def main(argv=None) -> int:
@@ -164,18 +214,26 @@ def main(argv=None) -> int:
     tr.add_argument("--iters", type=int, default=20000)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--out", default=DEFAULT_NET)
-    tr.add_argument("--eval-t", type=float, default=0.5)
+    tr.add_argument("--eval-seeds", default="1000,1001,1002")
+    tr.add_argument("--eval-times", default="0.3,0.5")
+    tr.add_argument("--kappas", default="0.25,0.5,1.0")
+    tr.add_argument("--kappa", type=float, default=0.5)
     tr.add_argument("--report", default=None)
     ev = sub.add_parser("evaluate")
     ev.add_argument("--net", default=DEFAULT_NET)
-    ev.add_argument("--seed", type=int, default=1000)
-    ev.add_argument("--t-end", type=float, default=0.5)
+    ev.add_argument("--eval-seeds", default="1000,1001,1002")
+    ev.add_argument("--eval-times", default="0.3,0.5")
+    ev.add_argument("--kappas", default="0.25,0.5,1.0")
     ev.add_argument("--report", default=None)
     a = ap.parse_args(argv)
     from . import hip
     log = lambda m: print(m, flush=True)
+    seeds = [int(v) for v in a.eval_seeds.split(",")]
+    times = [float(v) for v in a.eval_times.split(",")]
+    kappas = [float(v) for v in a.kappas.split(",")]
     if a.cmd == "train":
         net = hip.Net()
+        net.kappa = a.kappa
         t0 = time.time()
         collect(net, list(range(a.seeds)), a.n_fine, a.n_coarse, a.frames, a.t_end, log)
         t1 = time.time()
@@ -183,13 +241,25 @@ def main(argv=None) -> int:
         t2 = time.time()
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         net.save(a.out, seeds=a.seeds, frames=a.frames, n_fine=a.n_fine, n_coarse=a.n_coarse,
-                 iters=a.iters, samples=net.size(), loss=np.array(hist))
+                 iters=a.iters, samples=net.size(), loss=np.array(hist), noise_cells=NOISE_CELLS,
+                 t_end=a.t_end)
         log(f"collect {t1 - t0:.1f} s, train {t2 - t1:.1f} s, saved {a.out}")
         rep = {"collect_s": t1 - t0, "train_s": t2 - t1, "samples": net.size(), "loss": hist}
-        rep["eval"] = evaluate(net, t_end=a.eval_t, n_fine=a.n_fine, n_coarse=a.n_coarse, log=log)
+        rep["eval"] = evaluate(net, seeds, times, kappas, a.n_fine, a.n_coarse, log=log)
+        net.kappa = a.kappa
+        rep["one_step"] = [one_step_error(net, sd, t0, a.n_fine, a.n_coarse) for sd in seeds for t0 in (0.15, 0.3, 0.45)]
+        for e in rep["one_step"]:
+            log(f"one step seed {e['seed']} t0={e['t0']}: plain {np.round(e['plain'], 4).tolist()} "
+                f"corrected {np.round(e['corrected'], 4).tolist()}")
     else:
         net = hip.Net.load(a.net)
-        rep = evaluate(net, a.seed, a.t_end, log=log)
+        k0 = net.kappa
+        rep = evaluate(net, seeds, times, kappas, log=log)
+        net.kappa = k0
+        rep["one_step"] = [one_step_error(net, sd, t0) for sd in seeds for t0 in (0.15, 0.3, 0.45)]
+        for e in rep["one_step"]:
+            log(f"one step seed {e['seed']} t0={e['t0']}: plain {np.round(e['plain'], 4).tolist()} "
+                f"corrected {np.round(e['corrected'], 4).tolist()}")
     if a.report:
         with open(a.report, "w") as fh:
             json.dump(rep, fh, indent=1)

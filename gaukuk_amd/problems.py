@@ -101,16 +101,27 @@ def sod(nx: int = 256, **kw) -> Problem:
 
 
 def kelvin_helmholtz(n: int = 2048, ny: int | None = None, seed: int = 0, amp: float = 0.01,
-                     noise: float = 0.01, **kw) -> Problem:
+                     noise: float = 0.01, noise_cells: int | None = None, **kw) -> Problem:
     """Periodic KH on [0,1]^2: rho = 2, vx = +0.5 for |y-0.5| < 0.25, else rho = 1, vx = -0.5;
-    p = 2.5; vy = amp*sin(4 pi x) + noise*U(-1,1) (SplitMix64 keyed by j*nx + i)."""
+    p = 2.5; vy = amp*sin(4 pi x) + noise*U(-1,1) (SplitMix64 keyed by j*nx + i).  With
+    noise_cells = m (dividing n and ny) the noise is drawn on an m-wide grid of the same domain and
+    held constant over each of its cells, so grids of different resolution share one perturbation."""
     ny = n if ny is None else ny
     ext = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     x, y = _centres(ext, n, ny, 2)
     inner = np.abs(y - 0.5) < 0.25
     rho = np.where(inner, 2.0, 1.0)
     vx = np.where(inner, 0.5, -0.5)
-    idx = np.arange(n * ny, dtype=np.uint64).reshape(ny, n)
+    if noise_cells is None or noise_cells == n:
+        idx = np.arange(n * ny, dtype=np.uint64).reshape(ny, n)
+    else:
+        m = noise_cells
+        mx, my = m, m * ny // n
+        if n % m or ny % my or my * n != m * ny:
+            raise ValueError("noise_cells must divide the grid")
+        ci = np.arange(n) // (n // mx)
+        cj = np.arange(ny) // (ny // my)
+        idx = (cj[:, None] * mx + ci[None, :]).astype(np.uint64)
     vy = amp * np.sin(4.0 * np.pi * x) + noise * uniform_pm1(seed, idx)
     p = np.full_like(rho, 2.5)
     base = dict(name="kh", nx=n, ny=ny, extent=ext, integrator=3, rc_order=2, cfl=0.4,
