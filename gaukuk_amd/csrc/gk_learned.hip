@@ -1314,8 +1314,8 @@ gk_status gk_restrict(gk_ctx* fine, gk_ctx* coarse) {
     return refresh_cmax(coarse);
 }
 
-gk_status gk_net_collect(gk_net* n, gk_ctx* fine, gk_ctx* coarse, double* dt_coarse, long long* added) {
-    if (!n || !fine || !coarse) return GK_EINVAL;
+gk_status gk_net_collect(gk_net* n, gk_ctx* fine, gk_ctx* coarse, int steps, double* span, long long* added) {
+    if (!n || !fine || !coarse || steps < 1) return GK_EINVAL;
     if (fine->prec != 64 || coarse->prec != 64) return net_fail(n, GK_EINVAL, "collection runs both grids in FP64");
     if (fine->phys.integrator != coarse->phys.integrator || fine->phys.gamma != coarse->phys.gamma)
         return net_fail(n, GK_EINVAL, "fine and coarse grids must share integrator and gamma");
@@ -1346,13 +1346,14 @@ gk_status gk_net_collect(gk_net* n, gk_ctx* fine, gk_ctx* coarse, double* dt_coa
                        (const double*)coarse->buf[0], coarse->plane, coarse->pitch, coarse->nx, coarse->ny, bcv,
                        EosPolicy<double>::make(coarse->phys.gamma), n->feat, n->meta);
     GK_NET_HIP(n, hipGetLastError());
-    // one coarse step with recording: its flux and its dt D
+    // `steps` coarse steps with recording: their flux and their span D
+    const double tc0 = coarse->hclock->t;
     if ((s = sub(coarse, gk_flux_record(coarse, 1))) != GK_OK) return s;
-    if ((s = sub(coarse, gk_run_steps(coarse, 1))) != GK_OK) return s;
+    if ((s = sub(coarse, gk_run_steps(coarse, steps))) != GK_OK) return s;
     if ((s = sub(coarse, read_clock(coarse))) != GK_OK) return s;
     coarse->recording = false;
-    if (coarse->hclock->step != 1) return net_fail(n, GK_ESTATE, "coarse step did not run (stepmax?)");
-    const double D = coarse->hclock->dt;
+    if (coarse->hclock->step != steps) return net_fail(n, GK_ESTATE, "coarse steps did not run (stepmax?)");
+    const double D = coarse->hclock->t - tc0;
     // the fine run over exactly D, recorded
     if ((s = sub(fine, gk_flux_record(fine, 1))) != GK_OK) return s;
     double t0 = fine->hclock->t;
@@ -1373,7 +1374,7 @@ gk_status gk_net_collect(gk_net* n, gk_ctx* fine, gk_ctx* coarse, double* dt_coa
     GK_NET_HIP(n, hipGetLastError());
     GK_NET_HIP(n, hipStreamSynchronize(coarse->stream));
     n->n += nf;
-    if (dt_coarse) *dt_coarse = D;
+    if (span) *span = D;
     if (added) *added = nf;
     return GK_OK;
 }

@@ -150,7 +150,7 @@ def lib() -> C.CDLL:
         L.gk_net_last_error.restype = C.c_char_p
         L.gk_net_last_error.argtypes = [vp]
         L.gk_net_collect.restype = st
-        L.gk_net_collect.argtypes = [vp, vp, vp, dp, llp]
+        L.gk_net_collect.argtypes = [vp, vp, vp, ip, dp, llp]
         L.gk_flux_record.restype = st
         L.gk_flux_record.argtypes = [vp, ip]
         L.gk_flux_download.restype = st
@@ -453,9 +453,10 @@ class Net:
         self._check(self.L.gk_net_train(self.h, iters, blocks, tiles_per_wave, lr, seed, _fp(log)))
         return log[:iters]
 
-    def collect(self, fine: Solver, coarse: Solver):
+    def collect(self, fine: Solver, coarse: Solver, steps: int = 1):
+        """Append one sample per coarse face over `steps` coarse steps; returns (span, added)."""
         dt, added = C.c_double(), C.c_longlong()
-        self._check(self.L.gk_net_collect(self.h, fine.h, coarse.h, C.byref(dt), C.byref(added)))
+        self._check(self.L.gk_net_collect(self.h, fine.h, coarse.h, steps, C.byref(dt), C.byref(added)))
         return dt.value, added.value
 
     def save(self, path: str, **meta):

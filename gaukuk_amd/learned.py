@@ -62,7 +62,7 @@ def init_params(seed: int) -> np.ndarray:
 
 
 def collect(net, seeds, n_fine: int = 1024, n_coarse: int = 256, frames: int = 16, t_end: float = 0.6,
-            log=print) -> None:
+            log=print, window: int = 1) -> None:
     from . import hip
     f = n_fine // n_coarse
     nfc = (n_coarse + 1) * n_coarse * 2
@@ -77,7 +77,7 @@ def collect(net, seeds, n_fine: int = 1024, n_coarse: int = 256, frames: int = 1
             gap = tk - fine.t
             if gap > 1e-12:
                 fine.advance(gap)
-            net.collect(fine, coarse)
+            net.collect(fine, coarse, window)
         fine.close()
         coarse.close()
         log(f"seed {s}: {frames} frames, data set {net.size()} samples, {time.time() - t0:.1f} s")
@@ -218,6 +218,7 @@ def main(argv=None) -> int:
     tr.add_argument("--eval-times", default="0.3,0.5")
     tr.add_argument("--kappas", default="0.25,0.5,1.0")
     tr.add_argument("--kappa", type=float, default=0.5)
+    tr.add_argument("--window", type=int, default=1, help="coarse steps per target window")
     tr.add_argument("--report", default=None)
     ev = sub.add_parser("evaluate")
     ev.add_argument("--net", default=DEFAULT_NET)
@@ -235,14 +236,14 @@ def main(argv=None) -> int:
         net = hip.Net()
         net.kappa = a.kappa
         t0 = time.time()
-        collect(net, list(range(a.seeds)), a.n_fine, a.n_coarse, a.frames, a.t_end, log)
+        collect(net, list(range(a.seeds)), a.n_fine, a.n_coarse, a.frames, a.t_end, log, a.window)
         t1 = time.time()
         hist = fit(net, a.iters, a.lr, log=log)
         t2 = time.time()
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         net.save(a.out, seeds=a.seeds, frames=a.frames, n_fine=a.n_fine, n_coarse=a.n_coarse,
                  iters=a.iters, samples=net.size(), loss=np.array(hist), noise_cells=NOISE_CELLS,
-                 t_end=a.t_end)
+                 t_end=a.t_end, window=a.window)
         log(f"collect {t1 - t0:.1f} s, train {t2 - t1:.1f} s, saved {a.out}")
         rep = {"collect_s": t1 - t0, "train_s": t2 - t1, "samples": net.size(), "loss": hist}
         rep["eval"] = evaluate(net, seeds, times, kappas, a.n_fine, a.n_coarse, log=log)

@@ -173,10 +173,10 @@ const char* gk_net_last_error(const gk_net* net);
 
 /* Training data from a fine FP64 run (both grids FP64, same integrator and gamma, fine = f x coarse
  * in both axes): restrict the fine state onto the coarse grid, take the coarse grid's features at
- * every face, run one recorded coarse step (length D) and the fine run over exactly D, recorded;
- * append one sample per coarse face (target: restricted fine flux minus coarse flux, weight 0
- * where the shock sensor fires).  The fine run keeps going from t + D. */
-gk_status gk_net_collect(gk_net* net, gk_ctx* fine, gk_ctx* coarse, double* dt_coarse, long long* added);
+ * every face, run `steps` recorded coarse steps (span D) and the fine run over exactly D, recorded;
+ * append one sample per coarse face (target: restricted fine flux minus coarse flux, averaged over
+ * D; weight 0 where the shock sensor fires).  The fine run keeps going from t + D. */
+gk_status gk_net_collect(gk_net* net, gk_ctx* fine, gk_ctx* coarse, int steps, double* span, long long* added);
 /* Flux recording: while on, every stage adds (stage weight) * dt * flux at every face: mass, m_x,
  * m_y, E planes of (nx+1)*ny x faces then nx*(ny+1) y faces (face i between cells i-1 and i).
  * Turning it on zeroes the sums. */
