@@ -411,12 +411,16 @@ gk_status gk_advance(gk_ctx* c, double dtoutput, double* t, int* step) {
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, dtoutput);
     GK_TRY_HIP(c, hipGetLastError());
     int chunk = 4;
+    int last_step = -1;
     for (;;) {
         for (int k = 0; k < chunk; k++) enqueue_step(c);
         GK_TRY_HIP(c, hipGetLastError());
         gk_status s = read_clock(c);
         if (s != GK_OK) return s;
         if (!c->hclock->active) break;
+        // an active clock that did not move over a whole chunk would loop forever
+        if (c->hclock->step == last_step) return fail(c, GK_ESTATE, "device clock stopped advancing");
+        last_step = c->hclock->step;
         if (chunk < 256) chunk *= 2;
     }
     if (t) *t = c->hclock->t;
