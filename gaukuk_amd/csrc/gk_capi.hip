@@ -54,6 +54,13 @@ struct gk_ctx {
     int bc[4] = {0, 0, 0, 0};
     std::vector<StageSpec> stages;
     std::string err;
+    // point masses (gk_set_point_gravity)
+    int npoint = 0;
+    double pm[2][8] = {{0}};           // gm, x, y, z, vx, vy, vz, rs
+    double ab = 0, omega = 0, theta1 = 0;
+    double* xc = nullptr;              // device cell centres, ghost-inclusive
+    double* yc = nullptr;
+    double zc = 0;
     // learned flux correction (gk_learned.hip)
     gk_net* net = nullptr;                 // attached network: stages use the corrected kernel
     unsigned long long* lstats = nullptr;  // corrected / sensor / positivity face counts
@@ -134,6 +141,13 @@ static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
     a.dxRec = c->dxRec; a.dyRec = c->dyRec;
     a.eos = EosPolicy<R>::make(c->phys.gamma);
     a.clock = c->clock;
+    a.npoint = c->npoint;
+    for (int q = 0; q < 2; q++) {
+        double* d = q == 0 ? a.pm0 : a.pm1;
+        d[0] = c->pm[q][0]; d[1] = c->pm[q][1]; d[2] = c->pm[q][2]; d[3] = c->pm[q][3]; d[4] = c->pm[q][7];
+    }
+    a.ab = c->ab; a.omega = c->omega; a.theta1 = c->theta1;
+    a.xc = c->xc; a.yc = c->yc; a.zc = c->zc; a.ng = c->grid.nghost;
     return a;
 }
 
@@ -347,6 +361,8 @@ void gk_destroy(gk_ctx* c) {
     if (c->partial) (void)hipFree(c->partial);
     if (c->sums) (void)hipFree(c->sums);
     if (c->acc) (void)hipFree(c->acc);
+    if (c->xc) (void)hipFree(c->xc);
+    if (c->yc) (void)hipFree(c->yc);
     if (c->lstats) (void)hipFree(c->lstats);
     if (c->hclock) (void)hipHostFree(c->hclock);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -455,6 +471,41 @@ gk_status gk_step_log(gk_ctx* c, int* steps, double* t, double* dt, int cap, int
         if (dt) dt[k] = r.dt;
     }
     if (count) *count = n;
+    return GK_OK;
+}
+
+gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double z, double vx, double vy, double vz,
+                                double rs) {
+    if (!c) return GK_EINVAL;
+    // SourceTerm::EnrollPointGravity (src/source_term/source.cpp:33-67): the first mass gives point
+    // gravity, a second turns it into binary gravity, a third is an error
+    if (c->npoint >= 2) return fail(c, GK_EINVAL, "Setup failed: too many point gravity.");
+    if (!c->xc) {
+        // Domain cell centres (src/sim.cpp:37-39), ghost-inclusive index, computed as the
+        // reference computes them
+        const int g = c->grid.nghost, lx = c->nx + 2 * g, ly = c->ny + 2 * g;
+        const double dx = (c->grid.xmax - c->grid.xmin) / c->grid.nx;
+        const double dy = (c->grid.ymax - c->grid.ymin) / c->grid.ny;
+        const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
+        std::vector<double> hx(lx), hy(ly);
+        for (int i = 0; i < lx; i++) hx[i] = c->grid.xmin - g * dx + 0.5 * dx + i * dx;
+        for (int j = 0; j < ly; j++) hy[j] = c->grid.ymin - g * dy + 0.5 * dy + j * dy;
+        c->zc = c->grid.zmin - g * dz + 0.5 * dz + g * dz;   // the active plane k = nGhost
+        GK_TRY_HIP(c, hipMalloc(&c->xc, sizeof(double) * lx));
+        GK_TRY_HIP(c, hipMalloc(&c->yc, sizeof(double) * ly));
+        GK_TRY_HIP(c, hipMemcpy(c->xc, hx.data(), sizeof(double) * lx, hipMemcpyHostToDevice));
+        GK_TRY_HIP(c, hipMemcpy(c->yc, hy.data(), sizeof(double) * ly, hipMemcpyHostToDevice));
+    }
+    double* m = c->pm[c->npoint];
+    m[0] = gm; m[1] = x; m[2] = y; m[3] = z; m[4] = vx; m[5] = vy; m[6] = vz; m[7] = rs;
+    if (c->npoint == 1) {
+        const double* m0 = c->pm[0];
+        c->ab = std::sqrt((x - m0[1]) * (x - m0[1]) + (y - m0[2]) * (y - m0[2]) + (z - m0[3]) * (z - m0[3]));
+        c->omega = std::sqrt((m0[0] + gm) / (c->ab * c->ab * c->ab));
+        c->theta1 = std::atan2(y - m0[2], x - m0[1]);
+    }
+    c->npoint++;
+    c->gravity = true;
     return GK_OK;
 }
 

@@ -58,6 +58,14 @@ struct StageArgs {
     double dxRec, dyRec;
     EosPolicy<R> eos;
     DevClock* clock;
+    // point masses (SourceTerm::EnrollPointGravity): 0 none, 1 point gravity, 2 binary gravity
+    int npoint;
+    double pm0[5], pm1[5];       // gm, x, y, z, rs of each mass
+    double ab, omega, theta1;    // the binary's separation, angular rate and initial phase
+    const double* xc;            // cell centres, ghost-inclusive index (Domain, src/sim.cpp:37-39)
+    const double* yc;
+    double zc;                   // centre of the active z plane
+    int ng;
 };
 
 constexpr int kWave = 64;

@@ -58,6 +58,24 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 
     const double dt = clk->dt;
     const R dtdx = R(dt * a.dxRec), dtdy = R(dt * a.dyRec), h = R(0.5 * dt);
+    // sources of the two half-kicks (src/sim.cpp:85-91, src/source_term/source.cpp:12-21):
+    // constant gravity, then point or binary gravity at the source cell's centre; the binary's
+    // second mass sits at angle omega t + theta1 for t = t + dt/2 (first kick) and t + dt (second)
+    R bx1[2] = {R(0), R(0)}, by1[2] = {R(0), R(0)};
+    if (GRAV && a.npoint == 2) {
+        const double ts[2] = {clk->t + 0.5 * dt, clk->t + dt};
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const double th = a.omega * ts[q] + a.theta1;
+            bx1[q] = R(a.ab * cos(th));
+            by1[q] = R(a.ab * sin(th));
+        }
+    }
+    auto kick = [&](R u[5], int q, int i, int j) {
+        gravity_kick(u, a.gx, a.gy, a.gz, h);
+        if (a.npoint) point_kick(u, a.npoint, a.pm0, a.pm1, bx1[q], by1[q], h, R(a.xc[i + a.ng]),
+                                 R(a.yc[j + a.ng]), R(a.zc));
+    };
     const long long P = a.plane;
     const int pitch = a.pitch;
     // forward-Euler ping-pong: the current state alternates between the two buffers
@@ -91,7 +109,10 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     };
     // gravity half-kick on U^n, boundary transforms of ghost cells, ConsToPrim
     auto finish_row = [&](int r, R u[5], R wout[5]) {
-        if (KICK_IN) gravity_kick(u, a.gx, a.gy, a.gz, h);
+        if (KICK_IN) {
+            int kyy;
+            kick(u, 0, cs, bc_remap(r, a.ny, a.bc[2], a.bc[3], kyy));
+        }
         if (xedge) bc_transform_lanes(u, 1, kx);
         if (r < 0 || r >= a.ny) {
             int ky;
@@ -147,7 +168,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 #pragma unroll
         for (int n = 0; n < 5; n++) b2[n] = bs[n];
         // the base of every stage is U^n after the first gravity half-kick
-        if (KICK_IN || KICK_BASE) gravity_kick(b2, a.gx, a.gy, a.gz, h);
+        if (KICK_IN || KICK_BASE) kick(b2, 0, cown, r - 2);
         R res[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
@@ -156,7 +177,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             else res[n] = a.c1 * b2[n] - a.c2 * dF;
         }
         a.eos.floors(res);
-        if (GRAV && LAST) gravity_kick(res, a.gx, a.gy, a.gz, h);
+        if (GRAV && LAST) kick(res, 1, cown, r - 2);
         if (LAST) {
             const R ws = a.eos.wave_speed(res);
             cacc = owner ? smax(cacc, ws) : cacc;

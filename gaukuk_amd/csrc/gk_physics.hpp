@@ -271,4 +271,38 @@ __host__ __device__ __forceinline__ void gravity_kick(R u[5], R gx, R gy, R gz, 
     if (gz != R(0)) { u[4] += u[3] * gz * h; u[3] += u[0] * gz * h; }
 }
 
+// Point-mass gravity half-kick on one cell at (x, y, z): src/source_term/point_gravity.cpp:20-74
+// (one mass) and binary_gravity.cpp:29-96 (two masses, the second at (x1, y1, z1); the indirect
+// term is computed and then zeroed there, so it contributes 0 * x1).  Energy takes the three old
+// momenta, then the momenta are kicked.  m = {gm, x, y, z, rs}.
+template <class R>
+__host__ __device__ __forceinline__ void point_kick(R u[5], int npoint, const double m0[5], const double m1[5],
+                                                    R x1, R y1, R h, R x, R y, R z) {
+    R gxdt, gydt, gzdt;
+    const R gm0dt = R(m0[0]) * h, rs0 = R(m0[4]);
+    const R delx0 = x - R(m0[1]), dely0 = y - R(m0[2]), delz0 = z - R(m0[3]);
+    const R r0Sqr = delx0 * delx0 + dely0 * dely0 + delz0 * delz0;
+    const R r0SmoInv = qdiv(R(1.0), qsqrt(r0Sqr + rs0 * rs0));
+    const R g0 = -gm0dt * r0SmoInv * r0SmoInv * r0SmoInv;
+    if (npoint == 1) {
+        gxdt = g0 * delx0;
+        gydt = g0 * dely0;
+        gzdt = g0 * delz0;
+    } else {
+        const R gm1dt = R(m1[0]) * h, rs1 = R(m1[4]), z1 = R(m1[3]);
+        const R delx1 = x - x1, dely1 = y - y1, delz1 = z - z1;
+        const R r1Sqr = delx1 * delx1 + dely1 * dely1 + delz1 * delz1;
+        const R r1SmoInv = qdiv(R(1.0), qsqrt(r1Sqr + rs1 * rs1));
+        const R g1 = -gm1dt * r1SmoInv * r1SmoInv * r1SmoInv;
+        const R gIndir = R(0);
+        gxdt = g0 * delx0 + g1 * delx1 + gIndir * x1;
+        gydt = g0 * dely0 + g1 * dely1 + gIndir * y1;
+        gzdt = g0 * delz0 + g1 * delz1 + gIndir * z1;
+    }
+    u[4] += u[1] * gxdt + u[2] * gydt + u[3] * gzdt;
+    u[1] += u[0] * gxdt;
+    u[2] += u[0] * gydt;
+    u[3] += u[0] * gzdt;
+}
+
 }  // namespace gk

@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get("GK_LIB_PATH") or os.path.join(HERE, "libgaukuk_hip.so
 GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ESTATE", 5: "GK_ENUMERIC"}
 
 # Every symbol include/gaukuk.h declares (checked by tests/test_capi_symbols.py).
-SYMBOLS = ["gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
+SYMBOLS = ["gk_set_point_gravity", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
            "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version",
@@ -63,6 +63,8 @@ def lib() -> C.CDLL:
         L.gk_create.restype = st
         L.gk_create.argtypes = [C.POINTER(GkGrid), C.POINTER(GkPhys), C.POINTER(vp)]
         L.gk_destroy.argtypes = [vp]
+        L.gk_set_point_gravity.restype = st
+        L.gk_set_point_gravity.argtypes = [vp] + [C.c_double] * 8
         for name in ("gk_upload_cons", "gk_download_cons"):
             getattr(L, name).restype = st
             getattr(L, name).argtypes = [vp, vp]
@@ -203,6 +205,8 @@ class Solver:
         if st != 0:
             raise GaukukError(st, self.L.gk_last_error(None).decode())
         self.h = h
+        for m in getattr(prob, "point", ()):
+            self._check(self.L.gk_set_point_gravity(self.h, *[float(v) for v in m]))
         if seg_rows:
             self._check(self.L.gk_set_tile(self.h, seg_rows))
         if upload and prob.cons is not None:

@@ -1,6 +1,6 @@
 // setup_from_file.cpp -- active-region conserved state from the binary file named by
 // $GK_IC_FILE: int32 nvar (5), int32 nx, int32 ny, float64 cons[5][ny][nx]; config keys
-// gx, gy, gz enroll constant gravity.  Lets any array (e.g. from gaukuk_amd.problems) start a run.
+// gx, gy, gz enroll constant gravity; npoint and pg<q>_* enroll point masses.  Lets any array (e.g. from gaukuk_amd.problems) start a run.
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -35,6 +35,13 @@ void Sim::Setup() {
                 cons(v, grid.kb, grid.jb + j, grid.ib + i) = (Real)buf[((size_t)v * grid.ny + j) * grid.nx + i];
     Config& c = Config::getInstance();
     srcTerm.EnrollConstGravityVector(c.get("gx", 0.0), c.get("gy", 0.0), c.get("gz", 0.0));
+    // npoint (1 or 2) and pg<q>_gm, _x, _y, _z, _vx, _vy, _vz, _rs enroll point masses in order
+    const int npoint = (int)c.get("npoint", 0.0);
+    for (int q = 0; q < npoint; q++) {
+        auto get = [&](const char* name) { return (Real)c.get("pg" + std::to_string(q) + "_" + name, 0.0); };
+        srcTerm.EnrollPointGravity(get("gm"), get("x"), get("y"), get("z"), get("vx"), get("vy"), get("vz"),
+                                   get("rs"));
+    }
 }
 
 }  // namespace Gaukuk

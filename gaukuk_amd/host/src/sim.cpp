@@ -52,6 +52,10 @@ void Sim::EnsureDevice() {
         gk_status s = gk_create(&g, &p, &c);
         if (s != GK_OK) throw std::runtime_error(std::string("gk_create: ") + gk_last_error(nullptr));
         ctx_ = c;
+        for (int q = 0; q < srcTerm.nPoint; q++) {
+            const Real* m = srcTerm.mass[q];
+            Check(gk_set_point_gravity(ctx_, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7]), "gk_set_point_gravity");
+        }
     }
     if (hostDirty_) {
         Check(gk_upload_cons(ctx_, cons.data()), "gk_upload_cons");

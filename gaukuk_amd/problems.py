@@ -52,11 +52,23 @@ class Problem:
     dtoutput: float = 1e30
     bc: tuple = (1, 1, 1, 1)  # xleft, xright, yleft, yright
     g: tuple = (0.0, 0.0, 0.0)
+    # point masses, each (gm, x, y, z, vx, vy, vz, rs) as SourceTerm::EnrollPointGravity takes them
+    # (src/source_term/source.cpp:33-67); two masses = binary gravity
+    point: tuple = ()
     ng: int = 2
     cons: np.ndarray = field(default=None, repr=False)  # (5, ny, nx) float64
 
     def with_(self, **kw) -> "Problem":
         return replace(self, **kw)
+
+    def _point_keys(self) -> dict:
+        if not self.point:
+            return {}
+        keys = {"npoint": len(self.point)}
+        for q, m in enumerate(self.point):
+            for name, v in zip(("gm", "x", "y", "z", "vx", "vy", "vz", "rs"), m):
+                keys[f"pg{q}_{name}"] = v
+        return keys
 
     def in_values(self) -> dict:
         """Keys of the reference's `.in` grammar describing this problem."""
@@ -68,6 +80,7 @@ class Problem:
             "NX": self.nx, "NY": self.ny, "NZ": 1, "NGhost": self.ng,
             "xleft": self.bc[0], "xright": self.bc[1], "yleft": self.bc[2], "yright": self.bc[3],
             "zleft": 1, "zright": 1, "gx": self.g[0], "gy": self.g[1], "gz": self.g[2],
+            **self._point_keys(),
         }
 
 

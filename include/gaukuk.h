@@ -58,6 +58,14 @@ typedef struct {
     int precision;
 } gk_phys;
 
+/* SourceTerm::EnrollPointGravity (src/source_term/source.cpp:33-67): enroll a point mass (gm, position,
+ * velocity, smoothing length rs).  The first call enables point gravity
+ * (src/source_term/point_gravity.cpp), a second call turns it into binary gravity (the second mass
+ * orbits the first, src/source_term/binary_gravity.cpp), a third call is an error.  Applied at both
+ * Strang half-kicks after constant gravity (src/source_term/source.cpp:12-21). */
+gk_status gk_set_point_gravity(gk_ctx* ctx, double gm, double x, double y, double z, double vx, double vy, double vz,
+                               double rs);
+
 /* Sim::Sim() (src/sim.cpp:45-67): validate, allocate device state, select the integrator. */
 gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out);
 /* Sim::~Sim(): free everything owned by the context. */

@@ -19,7 +19,8 @@ class Params(C.Structure):
                 ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
                 ("gamma", C.c_double), ("cfl", C.c_double),
                 ("integrator", C.c_int), ("rc_order", C.c_int), ("stepmax", C.c_int),
-                ("bc", C.c_int * 4), ("g", C.c_double * 3)]
+                ("bc", C.c_int * 4), ("g", C.c_double * 3),
+                ("npoint", C.c_int), ("pg", (C.c_double * 8) * 2)]
 
 
 _libs = {}
@@ -67,6 +68,10 @@ def make_params(prob) -> Params:
         p.bc[k] = prob.bc[k]
     for k in range(3):
         p.g[k] = prob.g[k]
+    p.npoint = len(prob.point)
+    for q, m in enumerate(prob.point):
+        for k in range(8):
+            p.pg[q][k] = m[k]
     return p
 
 

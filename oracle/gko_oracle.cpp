@@ -29,7 +29,12 @@ struct Oracle {
     Real gamma, gm1Rec, cfl;
     Real dx, dy, dz, dxRec, dyRec, drmin;
     Real gx, gy, gz;
-    bool src_enrolled;
+    bool src_enrolled, has_const, has_point, has_binary;
+    // point masses (GravitySource, src/source_term/source.hpp:11-20) and the binary's orbit
+    struct Mass { Real gm, x, y, z, vx, vy, vz, rs; } obj0{}, obj1{};
+    Real ab_ = 0, omega_ = 0, theta1_ = 0;
+    std::vector<Real> xc, yc;   // cell centres (Domain, src/sim.cpp:37-39), ghost-inclusive index
+    Real zc0 = 0;               // z centre of the active plane
     // state
     std::vector<Real> cons, prim, flx1, flx2, consTemp;
     Real t, dt, dtUntilOutput, cmax;
@@ -58,7 +63,21 @@ struct Oracle {
         drmin = smin(smin(dx, dy), dz);
         // SourceTerm::EnrollConstGravityVector: src/source_term/source.cpp:23-31
         gx = (Real)p.g[0]; gy = (Real)p.g[1]; gz = (Real)p.g[2];
-        src_enrolled = (gx != 0 || gy != 0 || gz != 0);
+        has_const = (gx != 0 || gy != 0 || gz != 0);
+        has_point = has_binary = false;
+        src_enrolled = has_const;
+        // cell centres: xmin - nGhost*dx + 0.5*dx + i*dx (src/sim.cpp:37-39); the active z plane is
+        // k = nGhost of lenz = 1 + 2 nGhost
+        xc.resize(lenx); yc.resize(leny);
+        for (int i = 0; i < lenx; i++) xc[i] = (Real)p.xmin - (Real)ng * dx + (Real)0.5 * dx + (Real)i * dx;
+        for (int j = 0; j < leny; j++) yc[j] = (Real)p.ymin - (Real)ng * dy + (Real)0.5 * dy + (Real)j * dy;
+        zc0 = (Real)p.zmin - (Real)ng * dz + (Real)0.5 * dz + (Real)ng * dz;
+        // SourceTerm::EnrollPointGravity, called npoint times: src/source_term/source.cpp:33-67
+        for (int q = 0; q < p.npoint && q < 2; q++) {
+            const double* g = p.pg[q];
+            EnrollPointGravity((Real)g[0], (Real)g[1], (Real)g[2], (Real)g[3], (Real)g[4], (Real)g[5], (Real)g[6],
+                               (Real)g[7]);
+        }
         size_t n = (size_t)NV * leny * lenx;
         cons.assign(n, 0); prim.assign(n, 0); consTemp.assign(n, 0);
         flx1.assign((size_t)NV * leny * (lenx + 1), 0);
@@ -355,6 +374,87 @@ struct Oracle {
         }
     }
 
+    void EnrollPointGravity(Real gm, Real x, Real y, Real z, Real vx, Real vy, Real vz, Real rs) {
+        if (has_binary) throw std::runtime_error("Setup failed: too many point gravity.");
+        if (has_point) {
+            obj1 = {gm, x, y, z, vx, vy, vz, rs};
+            ab_ = std::sqrt((x - obj0.x) * (x - obj0.x) + (y - obj0.y) * (y - obj0.y) + (z - obj0.z) * (z - obj0.z));
+            omega_ = std::sqrt((obj0.gm + gm) / (ab_ * ab_ * ab_));
+            theta1_ = std::atan2(y - obj0.y, x - obj0.x);
+            has_binary = true;
+            has_point = false;
+        } else {
+            obj0 = {gm, x, y, z, vx, vy, vz, rs};
+            has_point = true;
+        }
+        src_enrolled = true;
+    }
+
+    // ---- Point-mass gravity kicks: src/source_term/point_gravity.cpp:11-93 and
+    //      binary_gravity.cpp:11-101 (energy with the old momenta, then the momenta) -----------
+    void PointGravity(std::vector<Real>& U, Real h) {
+        const Real gm = obj0.gm, x0 = obj0.x, y0 = obj0.y, z0 = obj0.z, rs = obj0.rs;
+        const Real gmdt = gm * h;
+#pragma omp parallel for schedule(static)
+        for (int j = jb; j < je; j++) {
+            for (int i = ib; i < ie; i++) {
+                const Real delx = xc[i] - x0, dely = yc[j] - y0, delz = zc0 - z0;
+                const Real r2 = delx * delx + dely * dely + delz * delz;
+                const Real rSmoInv = Real(1.0) / std::sqrt(r2 + rs * rs);
+                const Real gVecdtdr = -gmdt * rSmoInv * rSmoInv * rSmoInv;
+                const Real gxdt = gVecdtdr * delx, gydt = gVecdtdr * dely, gzdt = gVecdtdr * delz;
+                Real& d = U[at(DEN, j, i)];
+                Real& mx = U[at(MX, j, i)];
+                Real& my = U[at(MY, j, i)];
+                Real& mz = U[at(MZ, j, i)];
+                Real& e = U[at(ENG, j, i)];
+                e += mx * gxdt + my * gydt + mz * gzdt;
+                mx += d * gxdt;
+                my += d * gydt;
+                mz += d * gzdt;
+            }
+        }
+    }
+    void BinaryGravity(std::vector<Real>& U, Real tsrc, Real h) {
+        const Real gm0dt = obj0.gm * h, rs0 = obj0.rs, gm1dt = obj1.gm * h, rs1 = obj1.rs;
+        const Real theta = omega_ * tsrc + theta1_;
+        obj1.x = ab_ * std::cos(theta);
+        obj1.y = ab_ * std::sin(theta);
+        const Real x0 = obj0.x, y0 = obj0.y, z0 = obj0.z, x1 = obj1.x, y1 = obj1.y, z1 = obj1.z;
+        const Real gIndirdtdr1 = 0;   // the indirect term is computed and then zeroed (binary_gravity.cpp:44-45)
+#pragma omp parallel for schedule(static)
+        for (int j = jb; j < je; j++) {
+            for (int i = ib; i < ie; i++) {
+                const Real delx0 = xc[i] - x0, dely0 = yc[j] - y0, delz0 = zc0 - z0;
+                const Real delx1 = xc[i] - x1, dely1 = yc[j] - y1, delz1 = zc0 - z1;
+                const Real r0Sqr = delx0 * delx0 + dely0 * dely0 + delz0 * delz0;
+                const Real r0SmoInv = Real(1.0) / std::sqrt(r0Sqr + rs0 * rs0);
+                const Real g0 = -gm0dt * r0SmoInv * r0SmoInv * r0SmoInv;
+                const Real r1Sqr = delx1 * delx1 + dely1 * dely1 + delz1 * delz1;
+                const Real r1SmoInv = Real(1.0) / std::sqrt(r1Sqr + rs1 * rs1);
+                const Real g1 = -gm1dt * r1SmoInv * r1SmoInv * r1SmoInv;
+                const Real gxdt = g0 * delx0 + g1 * delx1 + gIndirdtdr1 * x1;
+                const Real gydt = g0 * dely0 + g1 * dely1 + gIndirdtdr1 * y1;
+                const Real gzdt = g0 * delz0 + g1 * delz1 + gIndirdtdr1 * z1;
+                Real& d = U[at(DEN, j, i)];
+                Real& mx = U[at(MX, j, i)];
+                Real& my = U[at(MY, j, i)];
+                Real& mz = U[at(MZ, j, i)];
+                Real& e = U[at(ENG, j, i)];
+                e += mx * gxdt + my * gydt + mz * gzdt;
+                mx += d * gxdt;
+                my += d * gydt;
+                mz += d * gzdt;
+            }
+        }
+    }
+    // SourceTerm::UpdateSource: src/source_term/source.cpp:12-21
+    void UpdateSource(std::vector<Real>& U, Real tsrc, Real h) {
+        if (has_const) ConstGravity(U, h);
+        if (has_point) PointGravity(U, h);
+        else if (has_binary) BinaryGravity(U, tsrc, h);
+    }
+
     // ---- Integrators: src/evolution/{forward_euler,rk2,rk3}.cpp ------------------------------
     void Stage(std::vector<Real>& in) {
         UpdateBD(in);
@@ -390,9 +490,10 @@ struct Oracle {
             CalCmax(cons, cmax);
             dt = cfl * drmin / cmax;
             dt = smin(dt, dtUntilOutput);
-            if (src_enrolled) ConstGravity(cons, Real(0.5) * dt);
+            // src/sim.cpp:85-91: half-kicks at t + dt/2 and t + dt
+            if (src_enrolled) UpdateSource(cons, t + Real(0.5) * dt, Real(0.5) * dt);
             Integrate();
-            if (src_enrolled) ConstGravity(cons, Real(0.5) * dt);
+            if (src_enrolled) UpdateSource(cons, t + dt, Real(0.5) * dt);
             t += dt;
             step++;
             taken++;

@@ -18,6 +18,9 @@
  *   EquationOfState              src/eos/adiabatic.cpp:12-198, eos.hpp:27-32
  *   Boundary types 0..3          src/boundary/{outflow_copy,periodic,reflective,outflow}.cpp
  *   SourceTerm::ConstGravity     src/source_term/const_gravity.cpp:10-67
+ *   SourceTerm::PointGravity     src/source_term/point_gravity.cpp:11-93
+ *   SourceTerm::BinaryGravity    src/source_term/binary_gravity.cpp:11-101
+ *   SourceTerm::UpdateSource / EnrollPointGravity   src/source_term/source.cpp:12-67
  *
  * Built with strict IEEE flags (-O2 -fno-fast-math -ffp-contract=off) it is
  * bitwise equal to the reference built the same way (golden fixtures under
@@ -44,6 +47,8 @@ typedef struct {
     int stepmax;               /* <= 0: unlimited */
     int bc[4];                 /* xleft, xright, yleft, yright: 0 copy, 1 periodic, 2 reflective, 3 outflow */
     double g[3];               /* constant gravity vector; zero vector = not enrolled */
+    int npoint;                /* point masses enrolled (0, 1: point gravity, 2: binary) */
+    double pg[2][8];           /* per mass: gm, x, y, z, vx, vy, vz, rs (EnrollPointGravity order) */
 } gko_params;
 
 typedef struct gko_sim gko_sim;

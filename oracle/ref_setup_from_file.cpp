@@ -7,7 +7,9 @@
 // conditions (the reference setups derive theirs from a truncated pi, src/gaukuk.hpp:19):
 //   int32 nvar(=5), int32 nx, int32 ny, float64 cons[5][ny][nx]
 // Optional config keys gx, gy, gz enroll constant gravity
-// (SourceTerm::EnrollConstGravityVector, src/source_term/source.cpp:23-31).
+// (SourceTerm::EnrollConstGravityVector, src/source_term/source.cpp:23-31); npoint (1 or 2) and
+// pg<q>_gm, _x, _y, _z, _vx, _vy, _vz, _rs enroll point masses in order
+// (SourceTerm::EnrollPointGravity, src/source_term/source.cpp:33-67; two masses = binary gravity).
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -41,6 +43,13 @@ void Sim::Setup() {
     Real gy = Config::getInstance().get("gy", 0.0);
     Real gz = Config::getInstance().get("gz", 0.0);
     srcTerm.EnrollConstGravityVector(gx, gy, gz);
+    const int npoint = (int)Config::getInstance().get("npoint", 0.0);
+    for (int q = 0; q < npoint; q++) {
+        auto key = [&](const char* name) { return "pg" + std::to_string(q) + "_" + name; };
+        auto get = [&](const char* name) { return (Real)Config::getInstance().get(key(name), 0.0); };
+        srcTerm.EnrollPointGravity(get("gm"), get("x"), get("y"), get("z"), get("vx"), get("vy"), get("vz"),
+                                   get("rs"));
+    }
 }
 
 }  // namespace Gaukuk

@@ -23,7 +23,9 @@ def load_run(name):
         gamma=m["gamma"], cfl=m["CFL"], integrator=int(m["integrator"]), rc_order=int(m["RcOrder"]),
         stepmax=int(m["stepmax"]), tmax=m["tmax"], dtoutput=m["dtoutput"],
         bc=(int(m["xleft"]), int(m["xright"]), int(m["yleft"]), int(m["yright"])),
-        g=(m["gx"], m["gy"], m["gz"]), ng=int(m["NGhost"]), cons=z["ic"].copy())
+        g=(m["gx"], m["gy"], m["gz"]), ng=int(m["NGhost"]), cons=z["ic"].copy(),
+        point=tuple(tuple(m[f"pg{q}_{k}"] for k in ("gm", "x", "y", "z", "vx", "vy", "vz", "rs"))
+                    for q in range(int(m.get("npoint", 0)))))
     return prob, z["final"].copy(), float(z["t"]), int(z["steps"])
 
 

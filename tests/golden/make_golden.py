@@ -53,6 +53,15 @@ def cases():
     out["blast48x56_mixed"] = mixed.with_(name="blast_mixed")
     out["rt24x64_rk3_gx"] = P.rayleigh_taylor(24, 64, stepmax=12).with_(g=(0.05, -0.1, 0.0), integrator=3,
                                                                         bc=(1, 1, 3, 2))
+    # point gravity (src/source_term/point_gravity.cpp): a smoothed mass below the active plane
+    # (z0 = 0 while the plane's centre is z = 0.5, so the z kick and m_z are exercised), outflow walls
+    out["point48_rk2_bc3"] = P.blast(48, 48, p_ratio=20.0, radius=0.15, center=(0.55, 0.45), bc=3,
+                                     stepmax=25).with_(integrator=2, point=((0.3, 0.5, 0.5, 0.0, 0, 0, 0, 0.1),))
+    # binary gravity (src/source_term/binary_gravity.cpp) on top of constant gravity, periodic, RK3
+    out["binary40x44_rk3_g"] = P.blast(40, 44, p_ratio=10.0, radius=0.1, center=(0.5, 0.5), bc=1,
+                                       stepmax=20).with_(integrator=3, g=(0.0, -0.05, 0.0),
+                                                         point=((0.4, 0.5, 0.5, 0.5, 0, 0, 0, 0.08),
+                                                                (0.1, 0.75, 0.5, 0.5, 0, 0, 0, 0.05)))
     return out
 
 
@@ -152,15 +161,24 @@ def vectors(rng):
 
 
 def main():
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default="", help="comma-separated run cases to (re)make; default: everything")
+    a = ap.parse_args()
+    only = [v for v in a.only.split(",") if v]
     if not os.path.exists(os.path.join(REF, "gaukuk_ref")):
         sys.exit("oracle/_ref/gaukuk_ref missing: run `make -C oracle ref` where the reference exists")
     for name, prob in cases().items():
+        if only and name not in only:
+            continue
         with tempfile.TemporaryDirectory() as d:
             final, t, steps, nout = run_reference(prob, d)
         meta = {k: v for k, v in prob.in_values().items()}
         np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), ic=prob.cons, final=final,
                             t=np.array(t), steps=np.array(steps), meta=json.dumps(meta))
         print(f"{name:28s} {prob.nx}x{prob.ny} steps={steps} t={t!r}")
+    if only:
+        return
     vectors(np.random.default_rng(20251203))
     print("vectors written")
     output_files()
