@@ -61,6 +61,10 @@ struct gk_ctx {
     double* xc = nullptr;              // device cell centres, ghost-inclusive
     double* yc = nullptr;
     double zc = 0;
+    // fixed (type 4) faces (gk_set_fixed_boundary): the saved state in the reference layout, and
+    // on the device the two ghost layers nearest each face (xl, xr, yl, yr) in the state precision
+    std::vector<double> fixed_ref;
+    void* fix[4] = {nullptr, nullptr, nullptr, nullptr};
     // learned flux correction (gk_learned.hip)
     gk_net* net = nullptr;                 // attached network: stages use the corrected kernel
     unsigned long long* lstats = nullptr;  // corrected / sensor / positivity face counts
@@ -84,6 +88,18 @@ static void record_launch(gk_ctx* c, size_t q);
 static gk_status fail(gk_ctx* c, gk_status s, const std::string& m) {
     c->err = m;
     return s;
+}
+
+static bool has_fixed_face(const gk_ctx* c) {
+    return c->bc[0] == 4 || c->bc[1] == 4 || c->bc[2] == 4 || c->bc[3] == 4;
+}
+
+// Boundary::FixedBDXL..FixedBDYR throw when no data was enrolled (src/boundary/fixed.cpp:18-19)
+static gk_status check_fixed(gk_ctx* c) {
+    if (has_fixed_face(c) && c->fixed_ref.empty())
+        return fail(c, GK_ESTATE,
+                    "fixed boundary is not initialized: call gk_set_fixed_boundary (EnrollFixedBoundaryData)");
+    return GK_OK;
 }
 
 static void plan_waves(gk_ctx* c, int seg_rows) {
@@ -148,21 +164,29 @@ static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
     }
     a.ab = c->ab; a.omega = c->omega; a.theta1 = c->theta1;
     a.xc = c->xc; a.yc = c->yc; a.zc = c->zc; a.ng = c->grid.nghost;
+    a.fxl = (const R*)c->fix[0]; a.fxr = (const R*)c->fix[1];
+    a.fyl = (const R*)c->fix[2]; a.fyr = (const R*)c->fix[3];
     return a;
 }
 
 using PlmLimiter = VanLeer;
 
-template <class R, class L, bool THREE>
-static void launch_stage_t(gk_ctx* c, const StageArgs<R>& a, bool last) {
+template <class R, class L, bool THREE, bool FIXED>
+static void launch_stage_f(gk_ctx* c, const StageArgs<R>& a, bool last) {
     dim3 grid(c->nblocks), block(kBlock);
     if (c->gravity) {
-        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, true>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, false>), grid, block, 0, c->stream, a);
+        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, true, FIXED>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, false, FIXED>), grid, block, 0, c->stream, a);
     } else {
-        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, true>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, false>), grid, block, 0, c->stream, a);
+        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, true, FIXED>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, false, FIXED>), grid, block, 0, c->stream, a);
     }
+}
+
+template <class R, class L, bool THREE>
+static void launch_stage_t(gk_ctx* c, const StageArgs<R>& a, bool last) {
+    if (has_fixed_face(c)) launch_stage_f<R, L, THREE, true>(c, a, last);
+    else launch_stage_f<R, L, THREE, false>(c, a, last);
 }
 
 template <class R>
@@ -250,14 +274,18 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
     for (int n = 0; n < 5; n++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) at(n, j + g, i + g) = (R)act[((size_t)n * ny + j) * nx + i];
-    // ghost fill over the active tangential range, x faces then y faces (src/boundary/boundary.cpp:127-142)
+    // ghost fill over the active tangential range, x faces then y faces (src/boundary/boundary.cpp:127-142);
+    // fixed faces copy the saved state at the same position (src/boundary/fixed.cpp:16-123)
+    auto saved = [&](int n, int j, int i) {
+        return (R)c->fixed_ref[(((size_t)n * lz + g) * ly + j) * lx + i];
+    };
     for (int j = g; j < g + ny; j++)
         for (int i = 0; i < lx; i++) {
             if (i >= g && i < g + nx) continue;
             int kind;
             const int si = bc_remap(i - g, nx, c->bc[0], c->bc[1], kind) + g;
             R u[5];
-            for (int n = 0; n < 5; n++) u[n] = at(n, j, si);
+            for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, j, si);
             bc_transform(u, 1, kind);
             for (int n = 0; n < 5; n++) at(n, j, i) = u[n];
         }
@@ -268,7 +296,7 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
                 int kind;
                 const int sj = bc_remap(j - g, ny, c->bc[2], c->bc[3], kind) + g;
                 R u[5];
-                for (int n = 0; n < 5; n++) u[n] = at(n, sj, i);
+                for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, sj, i);
                 bc_transform(u, 2, kind);
                 for (int n = 0; n < 5; n++) at(n, j, i) = u[n];
             }
@@ -289,8 +317,6 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
     if (grid->nghost < 2) { g_create_error = "NGhost must be >= 2 for the PLM stencil"; return GK_EINVAL; }
     if (grid->nx < 2) { g_create_error = "NX must be >= 2"; return GK_EINVAL; }
     if (phys->precision != 64 && phys->precision != 32) { g_create_error = "precision must be 64 or 32"; return GK_EINVAL; }
-    for (int f = 0; f < 4; f++)
-        if (phys->bc[f] == 4) { g_create_error = "fixed boundaries (type 4) are not supported"; return GK_EINVAL; }
     if (!(phys->gamma > 1.0)) { g_create_error = "gamma must exceed 1"; return GK_EINVAL; }
     if ((long long)grid->nx * grid->ny * 5 >= (1LL << 40)) { g_create_error = "grid too large"; return GK_EINVAL; }
 
@@ -301,7 +327,7 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
     c->esz = c->prec == 32 ? 4 : 8;
     c->nx = grid->nx; c->ny = grid->ny; c->pitch = grid->nx;
     c->plane = (long long)c->pitch * c->ny;
-    for (int f = 0; f < 4; f++) c->bc[f] = (phys->bc[f] >= 0 && phys->bc[f] <= 3) ? phys->bc[f] : 0;
+    for (int f = 0; f < 4; f++) c->bc[f] = (phys->bc[f] >= 0 && phys->bc[f] <= 4) ? phys->bc[f] : 0;
     if (c->ny == 1) { c->bc[2] = 0; c->bc[3] = 0; }  // y fluxes are multiplied by dyRec = 0
     c->gravity = phys->g[0] != 0 || phys->g[1] != 0 || phys->g[2] != 0;
     // Domain (src/sim.cpp:16-36)
@@ -364,6 +390,8 @@ void gk_destroy(gk_ctx* c) {
     if (c->xc) (void)hipFree(c->xc);
     if (c->yc) (void)hipFree(c->yc);
     if (c->lstats) (void)hipFree(c->lstats);
+    for (int k = 0; k < 4; k++)
+        if (c->fix[k]) (void)hipFree(c->fix[k]);
     if (c->hclock) (void)hipHostFree(c->hclock);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -424,6 +452,7 @@ gk_status gk_download_cons(gk_ctx* c, void* host) {
 gk_status gk_advance(gk_ctx* c, double dtoutput, double* t, int* step) {
     if (!c) return GK_EINVAL;
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    if (gk_status fs = check_fixed(c)) return fs;
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, dtoutput);
     GK_TRY_HIP(c, hipGetLastError());
     int chunk = 4;
@@ -509,9 +538,49 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
     return GK_OK;
 }
 
+gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host) {
+    if (!c || !host) return GK_EINVAL;
+    // Boundary::EnrollFixedBoundaryData: the ghost layers of the given state become the fixed faces'
+    // values; the stencil reads the two layers nearest each face
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = 1 + 2 * g;
+    const size_t total = 5 * (size_t)lx * ly * lz;
+    c->fixed_ref.resize(total);
+    for (size_t k = 0; k < total; k++)
+        c->fixed_ref[k] = c->prec == 32 ? (double)((const float*)host)[k] : ((const double*)host)[k];
+    auto ref = [&](int n, int j, int i) { return c->fixed_ref[(((size_t)n * lz + g) * ly + j) * lx + i]; };
+    std::vector<double> face[4];
+    face[0].resize(10 * (size_t)ny); face[1].resize(10 * (size_t)ny);
+    face[2].resize(10 * (size_t)nx); face[3].resize(10 * (size_t)nx);
+    for (int n = 0; n < 5; n++)
+        for (int d = 0; d < 2; d++) {
+            for (int j = 0; j < ny; j++) {
+                face[0][((size_t)n * ny + j) * 2 + d] = ref(n, j + g, g - 1 - d);
+                face[1][((size_t)n * ny + j) * 2 + d] = ref(n, j + g, g + nx + d);
+            }
+            for (int i = 0; i < nx; i++) {
+                face[2][((size_t)n * 2 + d) * nx + i] = ref(n, g - 1 - d, i + g);
+                face[3][((size_t)n * 2 + d) * nx + i] = ref(n, g + ny + d, i + g);
+            }
+        }
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    for (int f = 0; f < 4; f++) {
+        const size_t cnt = face[f].size();
+        if (!c->fix[f]) GK_TRY_HIP(c, hipMalloc(&c->fix[f], cnt * c->esz));
+        if (c->prec == 32) {
+            std::vector<float> tmp(face[f].begin(), face[f].end());
+            GK_TRY_HIP(c, hipMemcpy(c->fix[f], tmp.data(), cnt * 4, hipMemcpyHostToDevice));
+        } else {
+            GK_TRY_HIP(c, hipMemcpy(c->fix[f], face[f].data(), cnt * 8, hipMemcpyHostToDevice));
+        }
+    }
+    return GK_OK;
+}
+
 gk_status gk_run_steps(gk_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return GK_EINVAL;
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    if (gk_status fs = check_fixed(c)) return fs;
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
     for (int k = 0; k < nsteps; k++) enqueue_step(c);
     GK_TRY_HIP(c, hipGetLastError());
@@ -527,6 +596,7 @@ gk_status gk_synchronize(gk_ctx* c) {
 gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kernel_ms[4], int launches[4]) {
     if (!c || nsteps < 0) return GK_EINVAL;
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    if (gk_status fs = check_fixed(c)) return fs;
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
     // an event after every launch; the interval ending at a launch's event is its duration.

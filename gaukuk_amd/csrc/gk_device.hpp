@@ -62,6 +62,12 @@ struct StageArgs {
     int npoint;
     double pm0[5], pm1[5];       // gm, x, y, z, rs of each mass
     double ab, omega, theta1;    // the binary's separation, angular rate and initial phase
+    // fixed (type 4) faces: the two saved ghost layers nearest each face, by distance d from it:
+    // x faces [5][ny][2] (element (n, j, d)), y faces [5][2][nx] (element (n, d, i))
+    const R* fxl;
+    const R* fxr;
+    const R* fyl;
+    const R* fyr;
     const double* xc;            // cell centres, ghost-inclusive index (Domain, src/sim.cpp:37-39)
     const double* yc;
     double zc;                   // centre of the active z plane

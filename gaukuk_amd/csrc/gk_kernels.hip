@@ -42,11 +42,13 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 #endif
 
 // GRAV: constant gravity is enrolled; LAST: the stage ends the step (second half-kick, next
-// step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.
+// step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.  FIXED: a
+// face has type 4 (its ghost cells come from saved layers); a separate variant so that grids
+// without such faces run code with no trace of them.
 // One wave advances the cells of strip `strip`, segment `seg` of one grid by one stage.
 // `strip` and `seg` are wave-uniform (scalar registers), so row logic is scalar branches and
 // scalar address arithmetic.
-template <class R, class Lim, bool THREE, bool GRAV, bool LAST>
+template <class R, class Lim, bool THREE, bool GRAV, bool LAST, bool FIXED>
 __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane) {
     constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
     constexpr bool KICK_BASE = GRAV && THREE; // the combine base is U^n
@@ -99,19 +101,39 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     auto at = [&](const R* row, int n, unsigned off) -> const R& {
         return *reinterpret_cast<const R*>(reinterpret_cast<const char*>(row) + n * Pb + off);
     };
+    // fixed (type 4) x faces: only the edge strips hold their ghost lanes
+    const bool xfixed = FIXED && xedge && (a.bc[0] == 4 || a.bc[1] == 4);
     // raw conserved values of row r (ghost rows remapped; the row index is wave-uniform)
     auto load_row = [&](int r, R u[5]) {
         int ky;
         const int rs = bc_remap(r, a.ny, a.bc[2], a.bc[3], ky);
-        const R* row = in + (long long)rs * pitch;
+        if (FIXED && ky == 4) {
+            // a ghost row of a fixed y face: the saved layer rs of that face
+            const R* F = r < 0 ? a.fyl : a.fyr;
+            const long long pl = 2LL * a.nx, o = (long long)rs * a.nx + cs;
 #pragma unroll
-        for (int n = 0; n < 5; n++) u[n] = at(row, n, off_in);
+            for (int n = 0; n < 5; n++) u[n] = F[n * pl + o];
+        } else {
+            const R* row = in + (long long)rs * pitch;
+#pragma unroll
+            for (int n = 0; n < 5; n++) u[n] = at(row, n, off_in);
+        }
+        if (xfixed && kx == 4) {
+            // a ghost column of a fixed x face (cs is its distance from the face); corner cells
+            // are never read, so ghost rows take the nearest active row
+            const R* F = c < 0 ? a.fxl : a.fxr;
+            const long long pl = 2LL * a.ny, o = (long long)min(max(r, 0), a.ny - 1) * 2 + cs;
+#pragma unroll
+            for (int n = 0; n < 5; n++) u[n] = F[n * pl + o];
+        }
     };
     // gravity half-kick on U^n, boundary transforms of ghost cells, ConsToPrim
     auto finish_row = [&](int r, R u[5], R wout[5]) {
         if (KICK_IN) {
+            // ghosts copy kicked active cells, except those of fixed faces, which hold saved data
             int kyy;
-            kick(u, 0, cs, bc_remap(r, a.ny, a.bc[2], a.bc[3], kyy));
+            const int rs = bc_remap(r, a.ny, a.bc[2], a.bc[3], kyy);
+            if (!FIXED || (kyy != 4 && !(xfixed && kx == 4))) kick(u, 0, cs, rs);
         }
         if (xedge) bc_transform_lanes(u, 1, kx);
         if (r < 0 || r >= a.ny) {
@@ -258,7 +280,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 
 // One RK stage of one grid: wave w of the (XCD-remapped) grid takes strip w % strips, segment
 // w / strips.
-template <class R, class Lim, bool THREE, bool GRAV, bool LAST>
+template <class R, class Lim, bool THREE, bool GRAV, bool LAST, bool FIXED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GK_WAVES_PER_EU)))
 void stage_kernel(const StageArgs<R> a) {
     if (!a.clock->active) return;
@@ -266,7 +288,7 @@ void stage_kernel(const StageArgs<R> a) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
     if (w >= a.nwaves) return;
-    march<R, Lim, THREE, GRAV, LAST>(a, w % a.strips, w / a.strips, lane);
+    march<R, Lim, THREE, GRAV, LAST, FIXED>(a, w % a.strips, w / a.strips, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -325,7 +347,7 @@ void ens_stage_kernel(const EnsArgs<R> e) {
             a.dxRec = d.dxRec; a.dyRec = d.dyRec;
             a.clock = clk;
             const int local = item - d.item_base;
-            march<R, Lim, THREE, GRAV, LAST>(a, local % d.strips, local / d.strips, lane);
+            march<R, Lim, THREE, GRAV, LAST, false>(a, local % d.strips, local / d.strips, lane);
         }
         item = next;
     }

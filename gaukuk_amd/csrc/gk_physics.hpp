@@ -209,7 +209,7 @@ template <class R>
 __host__ __device__ __forceinline__ void bc_transform(R u[5], int comp, int kind) {
     if (kind == 1) {
         u[comp] = -u[comp];
-    } else if (kind >= 2) {
+    } else if (kind == 2 || kind == 3) {
         const R m = u[comp];
         const R mc = (kind == 2) ? smin(m, R(0.0)) : smax(m, R(0.0));
         const R d = smax(u[0], R(GK_DENSITY_FLOOR));
@@ -236,27 +236,31 @@ __host__ __device__ __forceinline__ void bc_transform_lanes(R u[5], int comp, in
     const R keNew = R(0.5) * inv * mc * mc;
     R eint = u[4] - keOld;
     eint = smax(eint, R(1e-16));
-    const bool outflow = kind >= 2;
+    const bool outflow = (kind == 2) | (kind == 3);
     u[4] = outflow ? eint + keNew : u[4];
     u[comp] = outflow ? mc : ((kind == 1) ? -m : m);
     u[0] = outflow ? d : u[0];
 }
 
 // Ghost index map for one axis of length n with face types (lo, hi): returns the source index
-// and sets kind (0 none, 1 reflect, 2 outflow-low, 3 outflow-high).  Unknown codes behave as
-// type 0, like the reference's registration fallback (src/boundary/boundary.cpp:27-29).
+// and sets kind (0 none, 1 reflect, 2 outflow-low, 3 outflow-high, 4 fixed).  For a fixed face
+// (type 4, src/boundary/fixed.cpp) the return value is the ghost's distance from the face (0 for
+// the nearest layer), an index into that face's saved layers.  Unknown codes behave as type 0,
+// like the reference's registration fallback (src/boundary/boundary.cpp:27-29).
 __host__ __device__ __forceinline__ int bc_remap(int k, int n, int lo, int hi, int& kind) {
     kind = 0;
     if (k < 0) {
         if (lo == 1) return k + n;
         if (lo == 2) { kind = 1; return -1 - k; }
         if (lo == 3) { kind = 2; return -1 - k; }
+        if (lo == 4) { kind = 4; return -1 - k; }
         return 0;
     }
     if (k >= n) {
         if (hi == 1) return k - n;
         if (hi == 2) { kind = 1; return 2 * n - 1 - k; }
         if (hi == 3) { kind = 3; return 2 * n - 1 - k; }
+        if (hi == 4) { kind = 4; return k - n; }
         return n - 1;
     }
     return k;

@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get("GK_LIB_PATH") or os.path.join(HERE, "libgaukuk_hip.so
 GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ESTATE", 5: "GK_ENUMERIC"}
 
 # Every symbol include/gaukuk.h declares (checked by tests/test_capi_symbols.py).
-SYMBOLS = ["gk_set_point_gravity", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
+SYMBOLS = ["gk_set_point_gravity", "gk_set_fixed_boundary", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
            "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version",
@@ -65,7 +65,7 @@ def lib() -> C.CDLL:
         L.gk_destroy.argtypes = [vp]
         L.gk_set_point_gravity.restype = st
         L.gk_set_point_gravity.argtypes = [vp] + [C.c_double] * 8
-        for name in ("gk_upload_cons", "gk_download_cons"):
+        for name in ("gk_upload_cons", "gk_download_cons", "gk_set_fixed_boundary"):
             getattr(L, name).restype = st
             getattr(L, name).argtypes = [vp, vp]
         for name in ("gk_upload_active", "gk_download_active"):
@@ -207,6 +207,11 @@ class Solver:
         self.h = h
         for m in getattr(prob, "point", ()):
             self._check(self.L.gk_set_point_gravity(self.h, *[float(v) for v in m]))
+        fixed = prob.fixed_full()
+        if fixed is not None:
+            full = np.zeros(self._ref_shape())
+            full[:, prob.ng] = fixed
+            self.set_fixed_boundary(full)
         if seg_rows:
             self._check(self.L.gk_set_tile(self.h, seg_rows))
         if upload and prob.cons is not None:
@@ -237,12 +242,23 @@ class Solver:
         self._check(self.L.gk_download_active(self.h, _dp(out)))
         return out
 
-    def download_ref_layout(self) -> np.ndarray:
+    def _ref_shape(self):
         ng = self.prob.ng
+        return (5, 1 + 2 * ng, self.prob.ny + 2 * ng, self.prob.nx + 2 * ng)
+
+    def download_ref_layout(self) -> np.ndarray:
         dt = np.float64 if self.precision == 64 else np.float32
-        out = np.empty((5, 1 + 2 * ng, self.prob.ny + 2 * ng, self.prob.nx + 2 * ng), dt)
+        out = np.empty(self._ref_shape(), dt)
         self._check(self.L.gk_download_cons(self.h, out.ctypes.data))
         return out
+
+    def set_fixed_boundary(self, ref: np.ndarray):
+        """Boundary::EnrollFixedBoundaryData: the ghost layers of `ref` (reference layout
+        [5][NZ+2g][NY+2g][NX+2g]) become the values of the type-4 faces."""
+        dt = np.float64 if self.precision == 64 else np.float32
+        a = np.ascontiguousarray(ref, dt)
+        assert a.shape == self._ref_shape()
+        self._check(self.L.gk_set_fixed_boundary(self.h, a.ctypes.data))
 
     def advance(self, dtoutput: float):
         t = C.c_double()

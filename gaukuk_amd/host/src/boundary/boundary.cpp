@@ -1,5 +1,5 @@
 // boundary.cpp -- host ghost fill by face type (semantics of the reference's
-// src/boundary/{outflow_copy,periodic,reflective,outflow}.cpp): each face is filled over the
+// src/boundary/{outflow_copy,periodic,reflective,outflow,fixed}.cpp): each face is filled over the
 // active range of the other axes only, so corners stay as they are (zero in fresh arrays);
 // z faces only when nz > 1 (src/boundary/boundary.cpp:127-142).
 #include <algorithm>
@@ -43,8 +43,12 @@ void Transform(Real u[5], int comp, int kind) {
 
 }  // namespace
 
-Boundary::BoundaryFunc Boundary::ForType(int axis, bool high, int code) {
-    return [axis, high, code](TArray<Real>& c, const Grid& g, const EquationOfState&) {
+Boundary::BoundaryFunc Boundary::ForType(int axis, bool high, int code, std::shared_ptr<const FixedState> fixed) {
+    return [axis, high, code, fixed](TArray<Real>& c, const Grid& g, const EquationOfState&) {
+        if (code == 4 && !(fixed && fixed->enrolled))
+            throw std::invalid_argument("fixed boundary is not initialized. call "
+                                        "'Boundary::EnrollFixedBoundaryData(const TArray<Real>& cons, "
+                                        "const Grid& grid)' in your setup");
         const int lo[3] = {g.ib, g.jb, g.kb}, hi[3] = {g.ie, g.je, g.ke}, n[3] = {g.nx, g.ny, g.nz};
         int from[3] = {g.ib, g.jb, g.kb}, to[3] = {g.ie, g.je, g.ke};
         // ghost layers along `axis`, active range along the other two
@@ -53,6 +57,10 @@ Boundary::BoundaryFunc Boundary::ForType(int axis, bool high, int code) {
         for (int k = from[2]; k < to[2]; k++)
             for (int j = from[1]; j < to[1]; j++)
                 for (int i = from[0]; i < to[0]; i++) {
+                    if (code == 4) {
+                        for (int v = 0; v < NVar; v++) c(v, k, j, i) = fixed->data(v, k, j, i);
+                        continue;
+                    }
                     int src[3] = {i, j, k};
                     const int here[3] = {i, j, k};
                     int kind;
@@ -70,16 +78,28 @@ Boundary::Boundary() {
     Config& c = Config::getInstance();
     for (int f = 0; f < 6; f++) {
         int code = (int)c.get(keys[f]);
-        if (code == 4) throw std::runtime_error("fixed boundaries (type 4) are not supported");
-        if (code < 0 || code > 3) code = 0;
+        if (code < 0 || code > 4) code = 0;
         type[f] = code;
     }
-    Bdxl = ForType(0, false, type[0]);
-    Bdxr = ForType(0, true, type[1]);
-    Bdyl = ForType(1, false, type[2]);
-    Bdyr = ForType(1, true, type[3]);
-    Bdzl = ForType(2, false, type[4]);
-    Bdzr = ForType(2, true, type[5]);
+    Bdxl = ForType(0, false, type[0], fixed_);
+    Bdxr = ForType(0, true, type[1], fixed_);
+    Bdyl = ForType(1, false, type[2], fixed_);
+    Bdyr = ForType(1, true, type[3], fixed_);
+    Bdzl = ForType(2, false, type[4], fixed_);
+    Bdzr = ForType(2, true, type[5], fixed_);
+}
+
+void Boundary::EnrollFixedBoundaryData(const TArray<Real>& cons, const Grid&, bool fixInner) {
+    if (fixInner) throw std::runtime_error("inner fixed regions have no device implementation in this build");
+    fixed_->data = cons;
+    fixed_->enrolled = true;
+    fixed_->version++;
+}
+
+bool Boundary::AnyFixed() const {
+    for (int f = 0; f < 6; f++)
+        if (type[f] == 4 && !selfDefined[f]) return true;
+    return false;
 }
 
 bool Boundary::AnySelfDefined() const {

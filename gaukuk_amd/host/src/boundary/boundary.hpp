@@ -1,12 +1,14 @@
 // boundary.hpp -- boundary-condition dispatch.  As in the reference (src/boundary/boundary.hpp:
 // 15-117, boundary.cpp:13-125) each face gets a function chosen by its type code from the
 // configuration: 0 copy the edge cell, 1 periodic, 2 reflective, 3 outflow (mirror, inflow
-// momentum clamped, energy adjusted); unknown codes fall back to 0.  On the device the same codes
+// momentum clamped, energy adjusted), 4 fixed (ghost layers restored from data the setup
+// enrolled); unknown codes fall back to 0.  On the device the same codes
 // select an index remap applied while the stage kernel loads its halo, so no ghost cells are
 // stored; the host functions here fill ghosts of host arrays (initial state and output files).
 #pragma once
 
 #include <functional>
+#include <memory>
 
 #include "../eos/eos.hpp"
 #include "../gaukuk.hpp"
@@ -36,7 +38,24 @@ public:
     void EnrollSelfDefineBDZR(BoundaryFunc f) { Bdzr = std::move(f); selfDefined[5] = true; }
     bool AnySelfDefined() const;
 
-    static BoundaryFunc ForType(int axis, bool high, int code);
+    // Fixed faces (src/boundary/fixed.cpp:242-269): keeps a copy of `cons`, whose ghost layers
+    // type-4 faces restore.  Inner fixed regions (fixInner, src/boundary/boundary.cpp:139-141) have
+    // no device implementation and are rejected.
+    void EnrollFixedBoundaryData(const TArray<Real>& cons, const Grid& grid, bool fixInner = false);
+    bool AnyFixed() const;
+    bool FixedEnrolled() const { return fixed_->enrolled; }
+    const TArray<Real>& FixedData() const { return fixed_->data; }
+    int FixedVersion() const { return fixed_->version; }
+
+    struct FixedState {
+        TArray<Real> data;
+        bool enrolled = false;
+        int version = 0;
+    };
+    static BoundaryFunc ForType(int axis, bool high, int code, std::shared_ptr<const FixedState> fixed = nullptr);
+
+private:
+    std::shared_ptr<FixedState> fixed_ = std::make_shared<FixedState>();
 };
 
 }  // namespace Gaukuk

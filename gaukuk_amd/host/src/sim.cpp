@@ -57,6 +57,11 @@ void Sim::EnsureDevice() {
             Check(gk_set_point_gravity(ctx_, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7]), "gk_set_point_gravity");
         }
     }
+    // fixed faces: the enrolled snapshot (re-sent when the setup enrolls new data)
+    if (boundary.AnyFixed() && boundary.FixedEnrolled() && fixedSent_ != boundary.FixedVersion()) {
+        Check(gk_set_fixed_boundary(ctx_, boundary.FixedData().data()), "gk_set_fixed_boundary");
+        fixedSent_ = boundary.FixedVersion();
+    }
     if (hostDirty_) {
         Check(gk_upload_cons(ctx_, cons.data()), "gk_upload_cons");
         hostDirty_ = false;

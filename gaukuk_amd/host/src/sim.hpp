@@ -62,6 +62,7 @@ private:
     gk_ctx* ctx_ = nullptr;
     bool hostDirty_ = true;   // host cons newer than the device copy
     bool hostStale_ = false;  // device newer than host cons
+    int fixedSent_ = 0;       // version of the fixed-face data the device holds
     bool verbose_ = true;
     void EnsureDevice();
     void Check(gk_status s, const char* what) const;

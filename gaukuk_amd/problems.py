@@ -55,16 +55,40 @@ class Problem:
     # point masses, each (gm, x, y, z, vx, vy, vz, rs) as SourceTerm::EnrollPointGravity takes them
     # (src/source_term/source.cpp:33-67); two masses = binary gravity
     point: tuple = ()
+    # fixed (type 4) boundaries: what the setup's ghost cells hold when it enrolls the fixed data
+    # (Boundary::EnrollFixedBoundaryData, src/boundary/fixed.cpp:242-269): one conserved state for
+    # every ghost cell (5 values) or a ghost-inclusive array [5][NY+2g][NX+2g]
+    fixed: tuple = ()
     ng: int = 2
     cons: np.ndarray = field(default=None, repr=False)  # (5, ny, nx) float64
 
     def with_(self, **kw) -> "Problem":
         return replace(self, **kw)
 
+    def fixed_full(self):
+        """The ghost-inclusive state [5][NY+2g][NX+2g] the setup enrolls as fixed-face data, or None."""
+        if len(self.fixed) == 0:
+            return None
+        shape = (5, self.ny + 2 * self.ng, self.nx + 2 * self.ng)
+        f = np.asarray(self.fixed, np.float64)
+        if f.ndim == 1:
+            return np.broadcast_to(f[:, None, None], shape).copy()
+        if f.shape != shape:
+            raise ValueError(f"fixed-face data has shape {f.shape}, expected {shape}")
+        return f.copy()
+
     def _point_keys(self) -> dict:
+        keys = {}
+        if len(self.fixed):
+            if np.ndim(self.fixed) == 1:
+                keys["fixedbd"] = 1
+                for q, v in enumerate(self.fixed):
+                    keys[f"fix_{q}"] = v
+            else:
+                keys["fixedbd"] = 2   # the array comes from the file named by $GK_FIXED_FILE
         if not self.point:
-            return {}
-        keys = {"npoint": len(self.point)}
+            return keys
+        keys["npoint"] = len(self.point)
         for q, m in enumerate(self.point):
             for name, v in zip(("gm", "x", "y", "z", "vx", "vy", "vz", "rs"), m):
                 keys[f"pg{q}_{name}"] = v

@@ -66,6 +66,12 @@ typedef struct {
 gk_status gk_set_point_gravity(gk_ctx* ctx, double gm, double x, double y, double z, double vx, double vy, double vz,
                                double rs);
 
+/* Boundary::EnrollFixedBoundaryData (src/boundary/fixed.cpp:242-269; faces FixedBDXL..FixedBDYR, fixed.cpp:16-123):
+ * `cons` is a state in the reference layout and the context's precision; its ghost layers become the
+ * values that type-4 faces restore before every stage.  Advancing a context with a type-4 face
+ * fails with GK_ESTATE until this is called, as the reference throws. */
+gk_status gk_set_fixed_boundary(gk_ctx* ctx, const void* cons);
+
 /* Sim::Sim() (src/sim.cpp:45-67): validate, allocate device state, select the integrator. */
 gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out);
 /* Sim::~Sim(): free everything owned by the context. */

@@ -44,6 +44,7 @@ def lib(flavour: str = "strict") -> C.CDLL:
         L.gko_is_continue.restype = ip; L.gko_is_continue.argtypes = [vp]
         L.gko_last_dt.restype = C.c_double; L.gko_last_dt.argtypes = [vp]
         L.gko_update_bd.argtypes = [vp]
+        L.gko_enroll_fixed.argtypes = [vp]
         L.gko_dt_log.restype = ip; L.gko_dt_log.argtypes = [vp, dp, dp, ip]
         L.gko_hllc.argtypes = [ip, dp, dp, ip, C.c_double, dp]
         L.gko_plm_row.argtypes = [ip, dp, dp, dp]
@@ -89,14 +90,30 @@ class OracleSim:
         ng = prob.ng
         self.shape = (5, prob.ny + 2 * ng, prob.nx + 2 * ng)
         full = np.zeros(self.shape, self.dtype)
+        fixed = prob.fixed_full()
+        if fixed is not None:
+            # a setup that fills the ghost cells, then enrolls the fixed data
+            full[:] = fixed
         full[:, ng:ng + prob.ny, ng:ng + prob.nx] = prob.cons
         self.L.gko_set_cons(self.h, full.ctypes.data)
+        if fixed is not None:
+            self.L.gko_enroll_fixed(self.h)
 
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
             self.L.gko_destroy(h)
             self.h = None
+
+    def enroll_fixed(self, full: np.ndarray):
+        """EnrollFixedBoundaryData with the ghost layers of `full` ([5][NY+2g][NX+2g]); the
+        current state is kept."""
+        cur = self.full()
+        snap = np.ascontiguousarray(full, self.dtype)
+        assert snap.shape == self.shape
+        self.L.gko_set_cons(self.h, snap.ctypes.data)
+        self.L.gko_enroll_fixed(self.h)
+        self.L.gko_set_cons(self.h, cur.ctypes.data)
 
     def full(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)

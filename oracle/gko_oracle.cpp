@@ -45,8 +45,6 @@ struct Oracle {
     explicit Oracle(const gko_params& p) : P(p) {
         nx = p.nx; ny = p.ny; ng = p.ng;
         if (nx <= 0 || ny <= 0 || ng < 0) throw std::runtime_error("Invalid grid parameters");
-        for (int f = 0; f < 4; f++)
-            if (p.bc[f] == 4) throw std::runtime_error("fixed boundary (type 4) is not restated");
         lenx = nx + 2 * ng; leny = ny + 2 * ng;
         ib = ng; ie = ib + nx; jb = ng; je = jb + ny;
         // EquationOfState ctor: src/eos/adiabatic.cpp:12-16
@@ -140,7 +138,7 @@ struct Oracle {
         cm = best;
     }
 
-    // ---- Boundaries (types 0..3), each face over the active tangential range ------------
+    // ---- Boundaries (types 0..4), each face over the active tangential range ------------
     // Type 0: src/boundary/outflow_copy.cpp:16-124; 1: periodic.cpp:19-131;
     // 2: reflective.cpp:19-166; 3: outflow.cpp:20-213.  Dispatch order Bdxl, Bdxr, Bdyl,
     // Bdyr: src/boundary/boundary.cpp:127-142 (z faces only when nz > 1: not here).
@@ -209,11 +207,32 @@ struct Oracle {
             }
         }
     }
+    // Type 4, fixed: src/boundary/fixed.cpp:16-127 -- the ghost layers saved by
+    // EnrollFixedBoundaryData (fixed.cpp:242-269) are copied back over the active tangential
+    // range.  Here the saved layers are a snapshot of the whole ghosted array.
+    std::vector<Real> fixedSnap;
+    void EnrollFixedBoundaryData() { fixedSnap = cons; }
+    void FillFixedX(std::vector<Real>& U, bool left) {
+        if (fixedSnap.empty())
+            throw std::invalid_argument("fixed boundary is not initialized");
+        const int gl = left ? 0 : ie, gr = left ? ib : lenx;
+        for (int n = 0; n < NV; n++)
+            for (int j = jb; j < je; j++)
+                for (int i = gl; i < gr; i++) U[at(n, j, i)] = fixedSnap[at(n, j, i)];
+    }
+    void FillFixedY(std::vector<Real>& U, bool left) {
+        if (fixedSnap.empty())
+            throw std::invalid_argument("fixed boundary is not initialized");
+        const int gl = left ? 0 : je, gr = left ? jb : leny;
+        for (int n = 0; n < NV; n++)
+            for (int j = gl; j < gr; j++)
+                for (int i = ib; i < ie; i++) U[at(n, j, i)] = fixedSnap[at(n, j, i)];
+    }
     void UpdateBD(std::vector<Real>& U) {
-        FillX(U, true, P.bc[0]);
-        FillX(U, false, P.bc[1]);
-        FillY(U, true, P.bc[2]);
-        FillY(U, false, P.bc[3]);
+        if (P.bc[0] == 4) FillFixedX(U, true); else FillX(U, true, P.bc[0]);
+        if (P.bc[1] == 4) FillFixedX(U, false); else FillX(U, false, P.bc[1]);
+        if (P.bc[2] == 4) FillFixedY(U, true); else FillY(U, true, P.bc[2]);
+        if (P.bc[3] == 4) FillFixedY(U, false); else FillY(U, false, P.bc[3]);
     }
 
     // ---- Reconstruction + HLLC ---------------------------------------------------------
@@ -549,6 +568,9 @@ double gko_time(const gko_sim* s) { return s->h.f ? (double)s->h.f->t : s->h.d->
 int gko_step(const gko_sim* s) { return s->h.f ? s->h.f->step : s->h.d->step; }
 int gko_is_continue(const gko_sim* s) { return s->h.f ? s->h.f->isContinue : s->h.d->isContinue; }
 double gko_last_dt(const gko_sim* s) { return s->h.f ? (double)s->h.f->dt : s->h.d->dt; }
+void gko_enroll_fixed(gko_sim* s) {
+    if (s->h.f) s->h.f->EnrollFixedBoundaryData(); else s->h.d->EnrollFixedBoundaryData();
+}
 void gko_update_bd(gko_sim* s) {
     if (s->h.f) s->h.f->UpdateBD(s->h.f->cons); else s->h.d->UpdateBD(s->h.d->cons);
 }

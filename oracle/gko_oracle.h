@@ -17,6 +17,7 @@
  *   Reconstruction PLM/1st order src/reconstruction/*.cpp, reconstruction.hpp:36-42
  *   EquationOfState              src/eos/adiabatic.cpp:12-198, eos.hpp:27-32
  *   Boundary types 0..3          src/boundary/{outflow_copy,periodic,reflective,outflow}.cpp
+ *   Boundary fixed (type 4)      src/boundary/fixed.cpp:16-127, 242-269
  *   SourceTerm::ConstGravity     src/source_term/const_gravity.cpp:10-67
  *   SourceTerm::PointGravity     src/source_term/point_gravity.cpp:11-93
  *   SourceTerm::BinaryGravity    src/source_term/binary_gravity.cpp:11-101
@@ -45,7 +46,7 @@ typedef struct {
     int integrator;            /* 1 forward Euler, 2 RK2, 3 RK3 (src/sim.cpp:60-66) */
     int rc_order;              /* 1 first order, 2 PLM */
     int stepmax;               /* <= 0: unlimited */
-    int bc[4];                 /* xleft, xright, yleft, yright: 0 copy, 1 periodic, 2 reflective, 3 outflow */
+    int bc[4];                 /* xleft, xright, yleft, yright: 0 copy, 1 periodic, 2 reflective, 3 outflow, 4 fixed */
     double g[3];               /* constant gravity vector; zero vector = not enrolled */
     int npoint;                /* point masses enrolled (0, 1: point gravity, 2: binary) */
     double pg[2][8];           /* per mass: gm, x, y, z, vx, vy, vz, rs (EnrollPointGravity order) */
@@ -65,6 +66,9 @@ double   gko_time(const gko_sim* s);
 int      gko_step(const gko_sim* s);
 int      gko_is_continue(const gko_sim* s);
 double   gko_last_dt(const gko_sim* s);
+/* Boundary::EnrollFixedBoundaryData (src/boundary/fixed.cpp:242-269): save the ghost layers of
+ * the current (set) state for type-4 faces. */
+void     gko_enroll_fixed(gko_sim* s);
 /* Boundary::UpdateBD on the current state (used before output). */
 void     gko_update_bd(gko_sim* s);
 /* Current step log: up to cap entries of (t, dt) for the last gko_advance. */

@@ -31,6 +31,9 @@ def run_from_file(workdir, prob, extra_args=(), in_values=None, timeout=600):
     inp = os.path.join(workdir, "case.in")
     write_in(inp, in_values if in_values is not None else prob.in_values())
     env = dict(os.environ, GK_IC_FILE=ic)
+    if np.ndim(prob.fixed) == 3:
+        env["GK_FIXED_FILE"] = os.path.join(workdir, "fixed.bin")
+        write_ic(env["GK_FIXED_FILE"], prob.fixed)
     r = subprocess.run([driver("from_file"), "-i", inp, *extra_args], cwd=workdir, env=env,
                        capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:

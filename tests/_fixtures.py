@@ -25,8 +25,18 @@ def load_run(name):
         bc=(int(m["xleft"]), int(m["xright"]), int(m["yleft"]), int(m["yright"])),
         g=(m["gx"], m["gy"], m["gz"]), ng=int(m["NGhost"]), cons=z["ic"].copy(),
         point=tuple(tuple(m[f"pg{q}_{k}"] for k in ("gm", "x", "y", "z", "vx", "vy", "vz", "rs"))
-                    for q in range(int(m.get("npoint", 0)))))
+                    for q in range(int(m.get("npoint", 0)))),
+        fixed=_fixed(z, m))
     return prob, z["final"].copy(), float(z["t"]), int(z["steps"])
+
+
+def _fixed(z, m):
+    kind = int(m.get("fixedbd", 0))
+    if kind == 1:
+        return tuple(m[f"fix_{q}"] for q in range(5))
+    if kind == 2:
+        return z["fixed_full"].copy()
+    return ()
 
 
 def load_vec(name):

@@ -62,6 +62,21 @@ def cases():
                                        stepmax=20).with_(integrator=3, g=(0.0, -0.05, 0.0),
                                                          point=((0.4, 0.5, 0.5, 0.5, 0, 0, 0, 0.08),
                                                                 (0.1, 0.75, 0.5, 0.5, 0, 0, 0, 0.05)))
+    # fixed (type 4) boundaries (src/boundary/fixed.cpp): every ghost cell starts at an inflow state
+    # (rho 1.2, vx 0.5, vy -0.1, p 0.3) and type-4 faces restore it at every fill; inflow on the left
+    # face and the top face, outflow right, reflecting bottom
+    fx = P.blast(48, 40, p_ratio=20.0, radius=0.12, center=(0.45, 0.4), bc=(4, 3, 2, 4), stepmax=30)
+    d0, vx0, vy0, p0 = 1.2, 0.5, -0.1, 0.3
+    fstate = (d0, d0 * vx0, d0 * vy0, 0.0, p0 / (fx.gamma - 1.0) + 0.5 * d0 * (vx0 * vx0 + vy0 * vy0))
+    out["fixed48x40_rk3"] = fx.with_(integrator=3, fixed=fstate)
+    # every ghost cell saved with its own value, gravity on: pins which saved layer each face
+    # restores, and that restored ghosts take no gravity kick
+    fv = P.blast(52, 44, p_ratio=15.0, radius=0.15, center=(0.5, 0.45), bc=(4, 4, 4, 4), stepmax=25)
+    rng = np.random.default_rng(5)
+    shp = (fv.ny + 2 * fv.ng, fv.nx + 2 * fv.ng)
+    fsnap = P.prim_to_cons(rng.uniform(0.8, 1.2, shp), rng.uniform(-0.3, 0.3, shp), rng.uniform(-0.3, 0.3, shp),
+                           np.zeros(shp), rng.uniform(0.2, 0.4, shp), fv.gamma)
+    out["fixedvar52x44_rk2_g"] = fv.with_(integrator=2, g=(0.0, -0.1, 0.0), fixed=fsnap)
     return out
 
 
@@ -73,6 +88,11 @@ def run_reference(prob, workdir):
     inp = os.path.join(workdir, "case.in")
     write_in(inp, prob.in_values())
     env = dict(os.environ, GK_IC_FILE=ic, OMP_NUM_THREADS="4")
+    if np.ndim(prob.fixed) == 3:
+        env["GK_FIXED_FILE"] = os.path.join(workdir, "fixed.bin")
+        with open(env["GK_FIXED_FILE"], "wb") as f:
+            np.array([5, prob.nx + 2 * prob.ng, prob.ny + 2 * prob.ng], np.int32).tofile(f)
+            np.ascontiguousarray(prob.fixed, np.float64).tofile(f)
     r = subprocess.run([os.path.join(REF, "gaukuk_ref"), "-i", inp], cwd=workdir, env=env,
                        capture_output=True, text=True, check=True)
     steps = int(re.search(r"total steps =\s*(\d+)", r.stdout).group(1))
@@ -174,8 +194,9 @@ def main():
         with tempfile.TemporaryDirectory() as d:
             final, t, steps, nout = run_reference(prob, d)
         meta = {k: v for k, v in prob.in_values().items()}
+        extra = {"fixed_full": prob.fixed} if np.ndim(prob.fixed) == 3 else {}
         np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), ic=prob.cons, final=final,
-                            t=np.array(t), steps=np.array(steps), meta=json.dumps(meta))
+                            t=np.array(t), steps=np.array(steps), meta=json.dumps(meta), **extra)
         print(f"{name:28s} {prob.nx}x{prob.ny} steps={steps} t={t!r}")
     if only:
         return
