@@ -65,6 +65,18 @@ struct gk_ctx {
     // on the device the two ghost layers nearest each face (xl, xr, yl, yr) in the state precision
     std::vector<double> fixed_ref;
     void* fix[4] = {nullptr, nullptr, nullptr, nullptr};
+    // inner fixed region: saved state (state layout) and per-row column intervals; bounding box
+    void* inner = nullptr;
+    int2* inner_rows = nullptr;
+    int inner_j0 = 0, inner_j1 = 0, inner_i0 = 0, inner_i1 = 0;
+    // immersed shell (gk_set_immersed_shell): the shell tables in device memory
+    int nshell = 0;
+    bool immersed = false;
+    long long* sh_cell = nullptr;
+    int* sh_ntar = nullptr;
+    long long* sh_tar = nullptr;
+    void* sh_dir = nullptr;
+    void* sh_gam = nullptr;
     // learned flux correction (gk_learned.hip)
     gk_net* net = nullptr;                 // attached network: stages use the corrected kernel
     unsigned long long* lstats = nullptr;  // corrected / sensor / positivity face counts
@@ -93,13 +105,51 @@ static gk_status fail(gk_ctx* c, gk_status s, const std::string& m) {
 static bool has_fixed_face(const gk_ctx* c) {
     return c->bc[0] == 4 || c->bc[1] == 4 || c->bc[2] == 4 || c->bc[3] == 4;
 }
+// boundaries beyond the face remaps: fixed faces, the inner fixed region, an immersed shell
+static bool has_extra_boundaries(const gk_ctx* c) { return has_fixed_face(c) || c->inner || c->immersed; }
 
 // Boundary::FixedBDXL..FixedBDYR throw when no data was enrolled (src/boundary/fixed.cpp:18-19)
 static gk_status check_fixed(gk_ctx* c) {
     if (has_fixed_face(c) && c->fixed_ref.empty())
         return fail(c, GK_ESTATE,
                     "fixed boundary is not initialized: call gk_set_fixed_boundary (EnrollFixedBoundaryData)");
+    // the shell is rewritten from the stage input in memory, while the first gravity half-kick is
+    // applied as the stage kernel loads U^n
+    if (c->immersed && c->gravity)
+        return fail(c, GK_EINVAL, "an immersed shell together with gravity is not supported");
     return GK_OK;
+}
+
+// Domain cell centres (src/sim.cpp:37-39), ghost-inclusive index, as the reference computes them
+static void domain_centres(const gk_ctx* c, std::vector<double>& hx, std::vector<double>& hy, double& zc) {
+    const int g = c->grid.nghost, lx = c->nx + 2 * g, ly = c->ny + 2 * g;
+    const double dx = (c->grid.xmax - c->grid.xmin) / c->grid.nx;
+    const double dy = (c->grid.ymax - c->grid.ymin) / c->grid.ny;
+    const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
+    hx.resize(lx); hy.resize(ly);
+    for (int i = 0; i < lx; i++) hx[i] = c->grid.xmin - g * dx + 0.5 * dx + i * dx;
+    for (int j = 0; j < ly; j++) hy[j] = c->grid.ymin - g * dy + 0.5 * dy + j * dy;
+    zc = c->grid.zmin - g * dz + 0.5 * dz + g * dz;   // the active plane k = nGhost
+}
+
+template <class R>
+static void launch_immersed_t(gk_ctx* c, void* u, void* alt, int pingpong, int always) {
+    ShellArgs<R> a;
+    a.u = (R*)u; a.alt = (R*)alt; a.pingpong = pingpong; a.always = always;
+    a.plane = c->plane; a.nshell = c->nshell;
+    a.cell = c->sh_cell; a.ntar = c->sh_ntar; a.tar = c->sh_tar;
+    a.dir = (const R*)c->sh_dir; a.gam = (const R*)c->sh_gam;
+    const EosPolicy<R> eos = EosPolicy<R>::make(c->phys.gamma);
+    a.gm1 = eos.gm1; a.gm1Rec = eos.gm1Rec;
+    a.clock = c->clock;
+    hipLaunchKernelGGL(immersed_kernel<R>, dim3((c->nshell + 255) / 256), dim3(256), 0, c->stream, a);
+}
+// the shell update of a stage's input (Boundary::UpdateBD, src/boundary/boundary.cpp:136-138)
+static void launch_immersed(gk_ctx* c, const StageSpec& s) {
+    if (!c->nshell) return;
+    const int pingpong = c->phys.integrator == 1;
+    if (c->prec == 32) launch_immersed_t<float>(c, c->buf[s.in], c->buf[s.out], pingpong, 0);
+    else launch_immersed_t<double>(c, c->buf[s.in], c->buf[s.out], pingpong, 0);
 }
 
 static void plan_waves(gk_ctx* c, int seg_rows) {
@@ -166,6 +216,7 @@ static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
     a.xc = c->xc; a.yc = c->yc; a.zc = c->zc; a.ng = c->grid.nghost;
     a.fxl = (const R*)c->fix[0]; a.fxr = (const R*)c->fix[1];
     a.fyl = (const R*)c->fix[2]; a.fyr = (const R*)c->fix[3];
+    a.inner = (const R*)c->inner; a.inner_rows = c->inner_rows;
     return a;
 }
 
@@ -185,7 +236,7 @@ static void launch_stage_f(gk_ctx* c, const StageArgs<R>& a, bool last) {
 
 template <class R, class L, bool THREE>
 static void launch_stage_t(gk_ctx* c, const StageArgs<R>& a, bool last) {
-    if (has_fixed_face(c)) launch_stage_f<R, L, THREE, true>(c, a, last);
+    if (has_fixed_face(c) || c->inner) launch_stage_f<R, L, THREE, true>(c, a, last);
     else launch_stage_f<R, L, THREE, false>(c, a, last);
 }
 
@@ -212,6 +263,7 @@ static void enqueue_step(gk_ctx* c) {
     hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
     for (size_t q = 0; q < c->stages.size(); q++) {
         if (c->recording) record_launch(c, q);
+        launch_immersed(c, c->stages[q]);
         launch_stage_any(c, c->stages[q]);
     }
     hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap,
@@ -304,6 +356,61 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
 }
 
 
+template <class T>
+static gk_status upload_vec(gk_ctx* c, T** dst, const std::vector<T>& v) {
+    if (!*dst) GK_TRY_HIP(c, hipMalloc((void**)dst, sizeof(T) * (v.empty() ? 1 : v.size())));
+    if (!v.empty()) GK_TRY_HIP(c, hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return GK_OK;
+}
+// a vector of doubles in the state precision
+static gk_status upload_real(gk_ctx* c, void** dst, const std::vector<double>& v) {
+    if (c->prec == 32) {
+        std::vector<float> f(v.begin(), v.end());
+        return upload_vec(c, (float**)dst, f);
+    }
+    return upload_vec(c, (double**)dst, v);
+}
+
+// Boundary::FixedBDInner's region (src/boundary/fixed.cpp:181-207): the central half-box of the
+// active cells, restricted to cell centres within 0.5 of the origin; per active row the region is
+// one run of columns (x^2 grows with |x| and rounding keeps that order)
+static gk_status set_inner(gk_ctx* c) {
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
+    const int ib = g, ie = g + nx, jb = g, je = g + ny;
+    const int il = static_cast<int>(ib + 0.25 * (ie - ib)), ir = static_cast<int>(ib + 0.75 * (ie - ib));
+    const int jl = static_cast<int>(jb + 0.25 * (je - jb)), jr = static_cast<int>(jb + 0.75 * (je - jb));
+    std::vector<double> xc, yc;
+    double zc;
+    domain_centres(c, xc, yc, zc);
+    std::vector<int2> rows((size_t)ny, make_int2(0, 0));
+    c->inner_j0 = ny; c->inner_j1 = 0; c->inner_i0 = nx; c->inner_i1 = 0;
+    for (int j = jl; j < jr; j++) {
+        int lo = -1, hi = -1;
+        for (int i = il; i < ir; i++) {
+            const double r2 = xc[i] * xc[i] + yc[j] * yc[j] + zc * zc;
+            if (r2 < 0.5 * 0.5) {
+                if (lo < 0) lo = i;
+                else if (hi != i) return fail(c, GK_EINVAL, "inner fixed region is not one run of columns per row");
+                hi = i + 1;
+            }
+        }
+        if (lo < 0) continue;
+        rows[j - g] = make_int2(lo - g, hi - g);
+        c->inner_j0 = std::min(c->inner_j0, j - g); c->inner_j1 = std::max(c->inner_j1, j - g + 1);
+        c->inner_i0 = std::min(c->inner_i0, lo - g); c->inner_i1 = std::max(c->inner_i1, hi - g);
+    }
+    std::vector<double> saved((size_t)c->plane * 5);
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = 1 + 2 * g;
+    for (int n = 0; n < 5; n++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+                saved[(size_t)n * c->plane + (size_t)j * c->pitch + i] =
+                    c->fixed_ref[(((size_t)n * lz + g) * ly + j + g) * lx + i + g];
+    if (gk_status s = upload_vec(c, &c->inner_rows, rows)) return s;
+    return upload_real(c, &c->inner, saved);
+}
+
+
 extern "C" {
 
 gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
@@ -392,6 +499,13 @@ void gk_destroy(gk_ctx* c) {
     if (c->lstats) (void)hipFree(c->lstats);
     for (int k = 0; k < 4; k++)
         if (c->fix[k]) (void)hipFree(c->fix[k]);
+    if (c->inner) (void)hipFree(c->inner);
+    if (c->inner_rows) (void)hipFree(c->inner_rows);
+    if (c->sh_cell) (void)hipFree(c->sh_cell);
+    if (c->sh_ntar) (void)hipFree(c->sh_ntar);
+    if (c->sh_tar) (void)hipFree(c->sh_tar);
+    if (c->sh_dir) (void)hipFree(c->sh_dir);
+    if (c->sh_gam) (void)hipFree(c->sh_gam);
     if (c->hclock) (void)hipHostFree(c->hclock);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -510,16 +624,9 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
     // gravity, a second turns it into binary gravity, a third is an error
     if (c->npoint >= 2) return fail(c, GK_EINVAL, "Setup failed: too many point gravity.");
     if (!c->xc) {
-        // Domain cell centres (src/sim.cpp:37-39), ghost-inclusive index, computed as the
-        // reference computes them
-        const int g = c->grid.nghost, lx = c->nx + 2 * g, ly = c->ny + 2 * g;
-        const double dx = (c->grid.xmax - c->grid.xmin) / c->grid.nx;
-        const double dy = (c->grid.ymax - c->grid.ymin) / c->grid.ny;
-        const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
-        std::vector<double> hx(lx), hy(ly);
-        for (int i = 0; i < lx; i++) hx[i] = c->grid.xmin - g * dx + 0.5 * dx + i * dx;
-        for (int j = 0; j < ly; j++) hy[j] = c->grid.ymin - g * dy + 0.5 * dy + j * dy;
-        c->zc = c->grid.zmin - g * dz + 0.5 * dz + g * dz;   // the active plane k = nGhost
+        std::vector<double> hx, hy;
+        domain_centres(c, hx, hy, c->zc);
+        const int lx = (int)hx.size(), ly = (int)hy.size();
         GK_TRY_HIP(c, hipMalloc(&c->xc, sizeof(double) * lx));
         GK_TRY_HIP(c, hipMalloc(&c->yc, sizeof(double) * ly));
         GK_TRY_HIP(c, hipMemcpy(c->xc, hx.data(), sizeof(double) * lx, hipMemcpyHostToDevice));
@@ -538,7 +645,7 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
     return GK_OK;
 }
 
-gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host) {
+gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host, int fix_inner) {
     if (!c || !host) return GK_EINVAL;
     // Boundary::EnrollFixedBoundaryData: the ghost layers of the given state become the fixed faces'
     // values; the stencil reads the two layers nearest each face
@@ -574,7 +681,137 @@ gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host) {
             GK_TRY_HIP(c, hipMemcpy(c->fix[f], face[f].data(), cnt * 8, hipMemcpyHostToDevice));
         }
     }
+    return fix_inner ? set_inner(c) : GK_OK;
+}
+
+gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, double rs) {
+    if (!c) return GK_EINVAL;
+    // Boundary::SetupImmersedShell (src/boundary/immersed.cpp:89-235), nz == 1
+    if (c->nx <= 1 || c->ny <= 1) return fail(c, GK_EINVAL, "immersed boundary has to be at least 2D. ");
+    const int g = c->grid.nghost, ib = g, ie = g + c->nx, jb = g, je = g + c->ny;
+    std::vector<double> xc, yc;
+    double zc;
+    domain_centres(c, xc, yc, zc);
+    const double dx = (c->grid.xmax - c->grid.xmin) / c->grid.nx;
+    const double dy = (c->grid.ymax - c->grid.ymin) / c->grid.ny;
+    const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
+    // first cell at or past the body's extent from the low end / last at or before it from the
+    // high end (FindLeftId, FindRightId: immersed.cpp:19-47)
+    bool outside = false;
+    auto first_from = [&](const std::vector<double>& cc, int b, int e, double centre, double lo) {
+        const double edge = centre - 1.2 * rs;
+        if (edge < lo) outside = true;
+        for (int i = b; i < e; i++)
+            if (cc[i] >= edge) return i;
+        return e;
+    };
+    auto last_to = [&](const std::vector<double>& cc, int b, int e, double centre, double hi) {
+        const double edge = centre + 1.2 * rs;
+        if (edge > hi) outside = true;
+        for (int i = e - 1; i >= b; i--)
+            if (cc[i] <= edge) return i;
+        return b;
+    };
+    const int il = first_from(xc, ib, ie, xs, c->grid.xmin), ir = last_to(xc, ib, ie, xs, c->grid.xmax);
+    const int jl = first_from(yc, jb, je, ys, c->grid.ymin), jr = last_to(yc, jb, je, ys, c->grid.ymax);
+    if (outside || il > ir || jl > jr) return fail(c, GK_EINVAL, "immersed boundary out of domain");
+    // shell cells: inside the body with a neighbour nGhost cells away (in one axis) outside it
+    // (IsShellCell: immersed.cpp:49-69)
+    const double sx = g * dx, sy = g * dy, sz = g * dz;
+    auto outside_at = [&](double ax, double ay, double az) { return std::sqrt(ax * ax + ay * ay + az * az) > rs; };
+    std::vector<int> si, sj;
+    for (int j = jl; j < jr; j++)
+        for (int i = il; i < ir; i++) {
+            const double ax = xc[i] - xs, ay = yc[j] - ys, az = zc - zs;
+            if (!(std::sqrt(ax * ax + ay * ay + az * az) < rs)) continue;
+            if (outside_at(ax + sx, ay, az) || outside_at(ax - sx, ay, az) || outside_at(ax, ay + sy, az) ||
+                outside_at(ax, ay - sy, az) || outside_at(ax, ay, az + sz) || outside_at(ax, ay, az - sz)) {
+                si.push_back(i); sj.push_back(j);
+            }
+        }
+    const int ns = (int)si.size();
+    std::vector<long long> cell(ns), tar((size_t)kShellTargets * ns, 0);
+    std::vector<int> ntar(ns);
+    std::vector<double> dir((size_t)4 * ns), gam((size_t)(kShellTargets + 1) * ns, 0.0);
+    const int reach = 2 * g;
+    for (int s = 0; s < ns; s++) {
+        const int i = si[s], j = sj[s];
+        cell[s] = (long long)(j - g) * c->pitch + (i - g);
+        const double ax = xc[i] - xs, ay = yc[j] - ys, az = zc - zs;
+        const double rnow = std::sqrt(ax * ax + ay * ay + az * az);
+        const double rxy = std::sqrt(ax * ax + ay * ay);
+        double cosp = 1.0, sinp = 0.0, cost = 1.0, sint = 0.0;
+        if (!(rxy < 1e-15)) { cosp = ax / rxy; sinp = ay / rxy; }
+        if (!(rnow < 1e-15)) { cost = az / rnow; sint = rxy / rnow; }
+        dir[s] = cosp; dir[ns + s] = sinp; dir[2 * ns + s] = cost; dir[3 * ns + s] = sint;
+        // mirror point of the cell through the surface, and the fluid cells within one cell of it
+        const double xi = sint * cosp * (2 * rs - rnow) + xs;
+        const double yi = sint * sinp * (2 * rs - rnow) + ys;
+        const double zi = cost * (2 * rs - rnow) + zs;
+        double dist[kShellTargets + 1];
+        dist[0] = 2 * (rs - rnow);
+        int nt = 0;
+        for (int di = -reach; di <= reach; di++)
+            for (int dj = -reach; dj <= reach; dj++) {
+                const int ti = i + di, tj = j + dj;
+                if (ti < ib || ti >= ie || tj < jb || tj >= je)
+                    return fail(c, GK_EINVAL, "immersed body is too close to the domain boundary, "
+                                              "neighboor search out of range.");
+                const double tx = xc[ti], ty = yc[tj], tz = zc;
+                if (!(std::abs(tx - xi) <= dx && std::abs(ty - yi) <= dy && std::abs(tz - zi) <= dz)) continue;
+                const double rt = std::sqrt((tx - xs) * (tx - xs) + (ty - ys) * (ty - ys) + (tz - zs) * (tz - zs));
+                if (!(rt > rs)) continue;
+                if (++nt > kShellTargets) return fail(c, GK_EINVAL, "n of imagine points is more than nTarMax");
+                tar[(size_t)(nt - 1) * ns + s] = (long long)(tj - g) * c->pitch + (ti - g);
+                dist[nt] = std::sqrt((tx - xi) * (tx - xi) + (ty - yi) * (ty - yi) + (tz - zi) * (tz - zi));
+            }
+        ntar[s] = nt;
+        // inverse-square distance weights normalised over the cell and its targets
+        double sum = 0.;
+        for (int n = 0; n <= nt; n++) sum = sum + 1. / (dist[n] * dist[n]);
+        for (int n = 0; n <= nt; n++) gam[(size_t)n * ns + s] = 1. / (dist[n] * dist[n] * sum);
+    }
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    for (void** p : {(void**)&c->sh_cell, (void**)&c->sh_ntar, (void**)&c->sh_tar, &c->sh_dir, &c->sh_gam})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    gk_status s;
+    if ((s = upload_vec(c, &c->sh_cell, cell)) || (s = upload_vec(c, &c->sh_ntar, ntar)) ||
+        (s = upload_vec(c, &c->sh_tar, tar)) || (s = upload_real(c, &c->sh_dir, dir)) ||
+        (s = upload_real(c, &c->sh_gam, gam)))
+        return s;
+    c->nshell = ns;
+    c->immersed = true;
     return GK_OK;
+}
+
+gk_status gk_update_bd(gk_ctx* c) {
+    if (!c) return GK_EINVAL;
+    if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    if (gk_status fs = check_fixed(c)) return fs;
+    // Boundary::UpdateBD's in-place parts on the current state (src/boundary/boundary.cpp:127-142):
+    // the immersed shell, then the inner fixed region; ghost cells live only in the stage loads
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    void* u = c->buf[current_buffer(c)];
+    if (c->nshell) {
+        if (c->prec == 32) launch_immersed_t<float>(c, u, u, 0, 1);
+        else launch_immersed_t<double>(c, u, u, 0, 1);
+    }
+    const int rows = c->inner_j1 - c->inner_j0, width = c->inner_i1 - c->inner_i0;
+    if (c->inner && rows > 0 && width > 0) {
+        const long long n = (long long)rows * width;
+        const dim3 grid((unsigned)((n + 255) / 256));
+        if (c->prec == 32)
+            hipLaunchKernelGGL(inner_reset_kernel<float>, grid, dim3(256), 0, c->stream, (float*)u,
+                               (const float*)c->inner, c->inner_rows, c->plane, c->pitch, c->inner_j0, rows,
+                               c->inner_i0, width);
+        else
+            hipLaunchKernelGGL(inner_reset_kernel<double>, grid, dim3(256), 0, c->stream, (double*)u,
+                               (const double*)c->inner, c->inner_rows, c->plane, c->pitch, c->inner_j0, rows,
+                               c->inner_i0, width);
+    }
+    GK_TRY_HIP(c, hipGetLastError());
+    return refresh_cmax(c);
 }
 
 gk_status gk_run_steps(gk_ctx* c, int nsteps) {
@@ -615,6 +852,10 @@ gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kerne
         hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
         rec(); kind.push_back(3);
         for (size_t q = 0; q < c->stages.size(); q++) {
+            if (c->nshell) {
+                launch_immersed(c, c->stages[q]);
+                rec(); kind.push_back(3);
+            }
             launch_stage_any(c, c->stages[q]);
             rec(); kind.push_back((int)q);
         }

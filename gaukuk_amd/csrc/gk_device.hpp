@@ -68,10 +68,34 @@ struct StageArgs {
     const R* fxr;
     const R* fyl;
     const R* fyr;
+    // inner fixed region (EnrollFixedBoundaryData with fixInner): the saved state (state layout)
+    // and per active row the column interval [x, y) of region cells (empty when x >= y)
+    const R* inner;
+    const int2* inner_rows;
     const double* xc;            // cell centres, ghost-inclusive index (Domain, src/sim.cpp:37-39)
     const double* yc;
     double zc;                   // centre of the active z plane
     int ng;
+};
+
+// Immersed reflective shell (Boundary::ImmersedReflective, src/boundary/immersed.cpp:237-300).
+// Cells are element offsets j * pitch + i of the state planes.
+constexpr int kShellTargets = 8;
+template <class R>
+struct ShellArgs {
+    R* u;                   // the state whose shell cells are rewritten
+    R* alt;                 // forward Euler: the other buffer (the clock's `cur` selects)
+    int pingpong;
+    int always;             // 1: run outside an advance (clock inactive)
+    long long plane;
+    int nshell;
+    const long long* cell;  // [nshell]
+    const int* ntar;        // [nshell]
+    const long long* tar;   // [kShellTargets][nshell]
+    const R* dir;           // [4][nshell]: cos phi, sin phi, cos theta, sin theta
+    const R* gam;           // [kShellTargets + 1][nshell]: distance weights, 0 = the shell cell
+    R gm1, gm1Rec;
+    const DevClock* clock;
 };
 
 constexpr int kWave = 64;

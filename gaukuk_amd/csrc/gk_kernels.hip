@@ -103,6 +103,19 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     };
     // fixed (type 4) x faces: only the edge strips hold their ghost lanes
     const bool xfixed = FIXED && xedge && (a.bc[0] == 4 || a.bc[1] == 4);
+    // inner fixed region (src/boundary/fixed.cpp:181-207): every read of a stage input or of U^n at
+    // a region cell returns the saved state, unkicked (the reference resets those cells after the
+    // face fills and after the first half-kick); results are stored as computed
+    auto inner_at = [&](int r) -> bool {
+        if (!FIXED || !a.inner || r < 0 || r >= a.ny) return false;
+        const int2 iv = a.inner_rows[r];
+        return c >= iv.x && c < iv.y;
+    };
+    auto inner_load = [&](int r, R u[5]) {
+        const long long o = (long long)r * pitch + c;
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = a.inner[n * P + o];
+    };
     // raw conserved values of row r (ghost rows remapped; the row index is wave-uniform)
     auto load_row = [&](int r, R u[5]) {
         int ky;
@@ -126,6 +139,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 #pragma unroll
             for (int n = 0; n < 5; n++) u[n] = F[n * pl + o];
         }
+        if (inner_at(r)) inner_load(r, u);
     };
     // gravity half-kick on U^n, boundary transforms of ghost cells, ConsToPrim
     auto finish_row = [&](int r, R u[5], R wout[5]) {
@@ -133,7 +147,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             // ghosts copy kicked active cells, except those of fixed faces, which hold saved data
             int kyy;
             const int rs = bc_remap(r, a.ny, a.bc[2], a.bc[3], kyy);
-            if (!FIXED || (kyy != 4 && !(xfixed && kx == 4))) kick(u, 0, cs, rs);
+            if (!FIXED || (kyy != 4 && !(xfixed && kx == 4) && !inner_at(r))) kick(u, 0, cs, rs);
         }
         if (xedge) bc_transform_lanes(u, 1, kx);
         if (r < 0 || r >= a.ny) {
@@ -190,7 +204,8 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 #pragma unroll
         for (int n = 0; n < 5; n++) b2[n] = bs[n];
         // the base of every stage is U^n after the first gravity half-kick
-        if (KICK_IN || KICK_BASE) kick(b2, 0, cown, r - 2);
+        if (KICK_IN || KICK_BASE)
+            if (!inner_at(r - 2)) kick(b2, 0, cown, r - 2);
         R res[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
@@ -219,6 +234,10 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         } else {
 #pragma unroll
             for (int n = 0; n < 5; n++) { bs[n] = at(in + o, n, off_own); ut[n] = R(0); }
+        }
+        if (inner_at(r - 2)) {
+            inner_load(r - 2, bs);
+            if (THREE) inner_load(r - 2, ut);
         }
     };
 
@@ -289,6 +308,73 @@ void stage_kernel(const StageArgs<R> a) {
     const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
     if (w >= a.nwaves) return;
     march<R, Lim, THREE, GRAV, LAST, FIXED>(a, w % a.strips, w / a.strips, lane);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Immersed reflective shell (Boundary::ImmersedReflective, src/boundary/immersed.cpp:237-300),
+// run on a stage's input before the stage (Boundary::UpdateBD applies it after the face fills,
+// src/boundary/boundary.cpp:136-138).  Each shell cell -- inside the body, within nGhost cells of
+// its surface -- takes the distance-weighted image of up to 8 fluid cells around its mirror
+// point, with the radial velocity reflected.  Shell cells lie inside the body and targets outside
+// it, so one thread per shell cell rewrites the state in place with no ordering.
+template <class R>
+__global__ __launch_bounds__(256) void immersed_kernel(const ShellArgs<R> a) {
+    if (!a.always && !a.clock->active) return;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nshell) return;
+    R* u = (a.pingpong && a.clock->cur) ? a.alt : a.u;
+    const long long P = a.plane;
+    const int ns = a.nshell;
+    const R cosp = a.dir[s], sinp = a.dir[ns + s], cost = a.dir[2 * ns + s], sint = a.dir[3 * ns + s];
+    const R g0 = a.gam[s];
+    const R wd = R(1.) / (R(1) - g0), wr = R(1.) / (R(-1.) - g0);
+    R dens = 0, vr = 0, vp = 0, vt = 0, pres = 0;
+    const int nt = a.ntar[s];
+    for (int n = 1; n <= nt; n++) {
+        const long long o = a.tar[(long long)(n - 1) * ns + s];
+        const R gn = a.gam[(long long)n * ns + s];
+        const R denTar = smax(u[o], R(GK_DENSITY_FLOOR));
+        const R denInv = R(1.0) / denTar;
+        dens += wd * gn * denTar;
+        const R vx = u[P + o] * denInv, vy = u[2 * P + o] * denInv, vz = u[3 * P + o] * denInv;
+        // Cartesian -> (radial, azimuthal, polar) about the body centre (CarToLocal, immersed.cpp:73-79)
+        const R vrl = vx * cosp * sint + vy * sinp * sint + vz * cost;
+        const R vpl = -vx * sinp + vy * cosp;
+        const R vtl = vx * cosp * cost + vy * sinp * cost - vz * sint;
+        vr += wr * gn * vrl;
+        vp += wd * gn * vpl;
+        vt += wd * gn * vtl;
+        const R engKin = R(0.5) * denTar * (vx * vx + vy * vy + vz * vz);
+        R preTar = a.gm1 * (u[4 * P + o] - engKin);
+        preTar = (preTar > R(GK_PRESSURE_FLOOR)) ? preTar : R(GK_PRESSURE_FLOOR);
+        pres += wd * gn * preTar;
+    }
+    // back to Cartesian (LocalToCar, immersed.cpp:82-87)
+    const R vx = vr * sint * cosp + vt * cost * cosp - vp * sinp;
+    const R vy = vr * sint * sinp + vt * cost * sinp + vp * cosp;
+    const R vz = vr * cost - vt * sint;
+    const long long c = a.cell[s];
+    u[c] = dens;
+    u[P + c] = dens * vx;
+    u[2 * P + c] = dens * vy;
+    u[3 * P + c] = dens * vz;
+    u[4 * P + c] = R(0.5) * dens * (vx * vx + vy * vy + vz * vz) + pres * a.gm1Rec;
+}
+
+// The inner fixed region written into a state (Boundary::FixedBDInner, src/boundary/fixed.cpp:
+// 181-207), for the boundary update outside the stages; rows [j0, j0 + nrows), one thread per
+// (row, column) of the region's bounding columns [i0, i0 + width).
+template <class R>
+__global__ __launch_bounds__(256) void inner_reset_kernel(R* u, const R* saved, const int2* rows, long long plane,
+                                                          int pitch, int j0, int nrows, int i0, int width) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)nrows * width) return;
+    const int j = j0 + (int)(k / width), i = i0 + (int)(k % width);
+    const int2 iv = rows[j];
+    if (i < iv.x || i >= iv.y) return;
+    const long long o = (long long)j * pitch + i;
+#pragma unroll
+    for (int n = 0; n < 5; n++) u[n * plane + o] = saved[n * plane + o];
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -1257,7 +1257,8 @@ gk_status gk_flux_record(gk_ctx* c, int on) {
     if (!c) return GK_EINVAL;
     if (!on) { c->recording = false; return GK_OK; }
     if (c->gravity) return fail(c, GK_EINVAL, "flux recording needs a grid without gravity");
-    if (has_fixed_face(c)) return fail(c, GK_EINVAL, "flux recording does not take fixed boundaries");
+    if (has_extra_boundaries(c))
+        return fail(c, GK_EINVAL, "flux recording takes face types 0-3 only (no fixed, inner or immersed boundaries)");
     if (c->ny < 2) return fail(c, GK_EINVAL, "flux recording needs a 2D grid");
     const long long nf = face_count(c);
     if (!c->acc) GK_TRY_HIP(c, hipMalloc(&c->acc, sizeof(double) * 4 * (size_t)nf));
@@ -1386,7 +1387,8 @@ gk_status gk_set_net(gk_ctx* c, gk_net* n) {
     if (!n) { c->net = nullptr; return GK_OK; }
     if (c->prec != 32) return fail(c, GK_EINVAL, "the corrected stage runs in precision 32");
     if (c->gravity) return fail(c, GK_EINVAL, "the corrected stage does not take gravity");
-    if (has_fixed_face(c)) return fail(c, GK_EINVAL, "the corrected stage does not take fixed boundaries");
+    if (has_extra_boundaries(c))
+        return fail(c, GK_EINVAL, "the corrected stage takes face types 0-3 only (no fixed, inner or immersed boundaries)");
     if (c->phys.rc_order == 1) return fail(c, GK_EINVAL, "the corrected stage uses PLM (RcOrder 2)");
     if (c->nx < kLT || c->ny < kLT) return fail(c, GK_EINVAL, "the corrected stage needs NX, NY >= 16");
     if (!c->lstats) {

@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get("GK_LIB_PATH") or os.path.join(HERE, "libgaukuk_hip.so
 GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ESTATE", 5: "GK_ENUMERIC"}
 
 # Every symbol include/gaukuk.h declares (checked by tests/test_capi_symbols.py).
-SYMBOLS = ["gk_set_point_gravity", "gk_set_fixed_boundary", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
+SYMBOLS = ["gk_set_point_gravity", "gk_set_fixed_boundary", "gk_set_immersed_shell", "gk_update_bd", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
            "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version",
@@ -65,7 +65,13 @@ def lib() -> C.CDLL:
         L.gk_destroy.argtypes = [vp]
         L.gk_set_point_gravity.restype = st
         L.gk_set_point_gravity.argtypes = [vp] + [C.c_double] * 8
-        for name in ("gk_upload_cons", "gk_download_cons", "gk_set_fixed_boundary"):
+        L.gk_set_fixed_boundary.restype = st
+        L.gk_set_fixed_boundary.argtypes = [vp, vp, ip]
+        L.gk_set_immersed_shell.restype = st
+        L.gk_set_immersed_shell.argtypes = [vp] + [C.c_double] * 4
+        L.gk_update_bd.restype = st
+        L.gk_update_bd.argtypes = [vp]
+        for name in ("gk_upload_cons", "gk_download_cons"):
             getattr(L, name).restype = st
             getattr(L, name).argtypes = [vp, vp]
         for name in ("gk_upload_active", "gk_download_active"):
@@ -185,7 +191,7 @@ def make_phys(prob, precision: int = 64) -> GkPhys:
         p.g[k] = prob.g[k]
     p.integrator, p.rc_order, p.stepmax = prob.integrator, prob.rc_order, prob.stepmax
     for k in range(4):
-        p.bc[k] = prob.bc[k]
+        p.bc[k] = prob.device_bc()[k]
     p.bc[4] = p.bc[5] = 1
     p.precision = precision
     return p
@@ -207,11 +213,13 @@ class Solver:
         self.h = h
         for m in getattr(prob, "point", ()):
             self._check(self.L.gk_set_point_gravity(self.h, *[float(v) for v in m]))
-        fixed = prob.fixed_full()
-        if fixed is not None:
+        enrolled = prob.enrolled_state()
+        if enrolled is not None:
             full = np.zeros(self._ref_shape())
-            full[:, prob.ng] = fixed
-            self.set_fixed_boundary(full)
+            full[:, prob.ng] = enrolled
+            self.set_fixed_boundary(full, prob.fix_inner)
+        if prob.shell:
+            self._check(self.L.gk_set_immersed_shell(self.h, *[float(v) for v in prob.shell]))
         if seg_rows:
             self._check(self.L.gk_set_tile(self.h, seg_rows))
         if upload and prob.cons is not None:
@@ -236,6 +244,13 @@ class Solver:
         a = np.ascontiguousarray(cons, np.float64)
         assert a.shape == (5, self.prob.ny, self.prob.nx)
         self._check(self.L.gk_upload_active(self.h, _dp(a)))
+        self._bd_done = False
+
+    def _begin(self):
+        # main's UpdateBD after Setup (src/main.cpp:30) changes active cells only through the
+        # immersed shell and the inner fixed region
+        if not getattr(self, "_bd_done", True) and (self.prob.shell or self.prob.fix_inner):
+            self.update_bd()
 
     def download(self) -> np.ndarray:
         out = np.empty((5, self.prob.ny, self.prob.nx), np.float64)
@@ -252,15 +267,23 @@ class Solver:
         self._check(self.L.gk_download_cons(self.h, out.ctypes.data))
         return out
 
-    def set_fixed_boundary(self, ref: np.ndarray):
+    def set_fixed_boundary(self, ref: np.ndarray, fix_inner: bool = False):
         """Boundary::EnrollFixedBoundaryData: the ghost layers of `ref` (reference layout
-        [5][NZ+2g][NY+2g][NX+2g]) become the values of the type-4 faces."""
+        [5][NZ+2g][NY+2g][NX+2g]) become the values of the type-4 faces; with fix_inner its active
+        cells in the inner region are restored before every stage."""
         dt = np.float64 if self.precision == 64 else np.float32
         a = np.ascontiguousarray(ref, dt)
         assert a.shape == self._ref_shape()
-        self._check(self.L.gk_set_fixed_boundary(self.h, a.ctypes.data))
+        self._check(self.L.gk_set_fixed_boundary(self.h, a.ctypes.data, int(bool(fix_inner))))
+
+    def update_bd(self):
+        """Boundary::UpdateBD's in-place part on the current state (immersed shell, inner region),
+        as the reference's main does once after Setup; advance() does it before the first step."""
+        self._check(self.L.gk_update_bd(self.h))
+        self._bd_done = True
 
     def advance(self, dtoutput: float):
+        self._begin()
         t = C.c_double()
         s = C.c_int()
         self._check(self.L.gk_advance(self.h, dtoutput, C.byref(t), C.byref(s)))
@@ -298,6 +321,7 @@ class Solver:
         return steps[:n.value], t[:n.value], dt[:n.value]
 
     def run_steps(self, n: int):
+        self._begin()
         self._check(self.L.gk_run_steps(self.h, n))
 
     def synchronize(self):

@@ -1,7 +1,8 @@
 // boundary.cpp -- host ghost fill by face type (semantics of the reference's
 // src/boundary/{outflow_copy,periodic,reflective,outflow,fixed}.cpp): each face is filled over the
 // active range of the other axes only, so corners stay as they are (zero in fresh arrays);
-// z faces only when nz > 1 (src/boundary/boundary.cpp:127-142).
+// z faces only when nz > 1 (src/boundary/boundary.cpp:127-142).  The immersed shell and the inner
+// fixed region rewrite active cells; that part runs on the device and UpdateBD marks it pending.
 #include <algorithm>
 #include <stdexcept>
 
@@ -90,10 +91,28 @@ Boundary::Boundary() {
 }
 
 void Boundary::EnrollFixedBoundaryData(const TArray<Real>& cons, const Grid&, bool fixInner) {
-    if (fixInner) throw std::runtime_error("inner fixed regions have no device implementation in this build");
     fixed_->data = cons;
     fixed_->enrolled = true;
+    fixed_->inner = fixInner;
     fixed_->version++;
+}
+
+void Boundary::SetupImmersedShell(const Grid& grid, const Domain&, Real xShell, Real yShell, Real zShell,
+                                  Real rShell) {
+    if (grid.nx <= 1 || grid.ny <= 1) throw std::invalid_argument("immersed boundary has to be at least 2D. ");
+    hasShell_ = true;
+    shell_[0] = xShell; shell_[1] = yShell; shell_[2] = zShell; shell_[3] = rShell;
+}
+
+const Boundary::BoundaryFunc& Boundary::Face(int f) const {
+    switch (f) {
+        case 0: return Bdxl;
+        case 1: return Bdxr;
+        case 2: return Bdyl;
+        case 3: return Bdyr;
+        case 4: return Bdzl;
+        default: return Bdzr;
+    }
 }
 
 bool Boundary::AnyFixed() const {
@@ -117,6 +136,7 @@ void Boundary::UpdateBD(TArray<Real>& cons, const Grid& grid, const Domain&, con
         Bdzl(cons, grid, eos);
         Bdzr(cons, grid, eos);
     }
+    if (hasShell_ || fixed_->inner) pending_ = true;
 }
 
 }  // namespace Gaukuk

@@ -5,6 +5,8 @@
 // enrolled); unknown codes fall back to 0.  On the device the same codes
 // select an index remap applied while the stage kernel loads its halo, so no ghost cells are
 // stored; the host functions here fill ghosts of host arrays (initial state and output files).
+// The in-place parts of a boundary update (immersed shell, inner fixed region) run on the device:
+// UpdateBD marks them pending and the Sim applies them to the uploaded state.
 #pragma once
 
 #include <functional>
@@ -28,8 +30,22 @@ public:
 
     void UpdateBD(TArray<Real>& cons, const Grid& grid, const Domain& domain, const EquationOfState& eos);
 
-    // Self-defined faces (setup-file hooks, src/boundary/boundary.hpp:28-45).  The device path has
-    // no user ghost-fill kernel: Sim::Advance rejects a run with a self-defined face.
+    // Self-defined faces (setup-file hooks, src/boundary/boundary.hpp:28-45): the setup file defines
+    // the static member for a face and enrolls it by name, or passes any function.  On the device a
+    // self-defined face whose ghost values do not depend on the state (an inflow) runs as a fixed
+    // face; the Sim checks that on a perturbed state and rejects any other.
+    static void SelfDefineBDXL(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
+    static void SelfDefineBDXR(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
+    static void SelfDefineBDYL(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
+    static void SelfDefineBDYR(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
+    static void SelfDefineBDZL(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
+    static void SelfDefineBDZR(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
+    void EnrollSelfDefineBDXL() { EnrollSelfDefineBDXL(&SelfDefineBDXL); }
+    void EnrollSelfDefineBDXR() { EnrollSelfDefineBDXR(&SelfDefineBDXR); }
+    void EnrollSelfDefineBDYL() { EnrollSelfDefineBDYL(&SelfDefineBDYL); }
+    void EnrollSelfDefineBDYR() { EnrollSelfDefineBDYR(&SelfDefineBDYR); }
+    void EnrollSelfDefineBDZL() { EnrollSelfDefineBDZL(&SelfDefineBDZL); }
+    void EnrollSelfDefineBDZR() { EnrollSelfDefineBDZR(&SelfDefineBDZR); }
     void EnrollSelfDefineBDXL(BoundaryFunc f) { Bdxl = std::move(f); selfDefined[0] = true; }
     void EnrollSelfDefineBDXR(BoundaryFunc f) { Bdxr = std::move(f); selfDefined[1] = true; }
     void EnrollSelfDefineBDYL(BoundaryFunc f) { Bdyl = std::move(f); selfDefined[2] = true; }
@@ -37,6 +53,17 @@ public:
     void EnrollSelfDefineBDZL(BoundaryFunc f) { Bdzl = std::move(f); selfDefined[4] = true; }
     void EnrollSelfDefineBDZR(BoundaryFunc f) { Bdzr = std::move(f); selfDefined[5] = true; }
     bool AnySelfDefined() const;
+    const BoundaryFunc& Face(int f) const;
+
+    // Immersed reflective sphere (src/boundary/immersed.cpp:89-235): recorded here, set up and
+    // applied on the device (gk_set_immersed_shell).
+    void SetupImmersedShell(const Grid& grid, const Domain& domain, Real xShell, Real yShell, Real zShell,
+                            Real rShell);
+    bool HasShell() const { return hasShell_; }
+    const double* Shell() const { return shell_; }
+    // an UpdateBD whose in-place part (shell, inner region) the device has not applied yet
+    bool DeviceUpdatePending() const { return pending_; }
+    void ClearDeviceUpdate() { pending_ = false; }
 
     // Fixed faces (src/boundary/fixed.cpp:242-269): keeps a copy of `cons`, whose ghost layers
     // type-4 faces restore.  Inner fixed regions (fixInner, src/boundary/boundary.cpp:139-141) have
@@ -44,18 +71,23 @@ public:
     void EnrollFixedBoundaryData(const TArray<Real>& cons, const Grid& grid, bool fixInner = false);
     bool AnyFixed() const;
     bool FixedEnrolled() const { return fixed_->enrolled; }
+    bool FixedInner() const { return fixed_->inner; }
     const TArray<Real>& FixedData() const { return fixed_->data; }
     int FixedVersion() const { return fixed_->version; }
 
     struct FixedState {
         TArray<Real> data;
         bool enrolled = false;
+        bool inner = false;
         int version = 0;
     };
     static BoundaryFunc ForType(int axis, bool high, int code, std::shared_ptr<const FixedState> fixed = nullptr);
 
 private:
     std::shared_ptr<FixedState> fixed_ = std::make_shared<FixedState>();
+    bool hasShell_ = false;
+    double shell_[4] = {0, 0, 0, 0};
+    bool pending_ = false;
 };
 
 }  // namespace Gaukuk

@@ -2,7 +2,11 @@
 // $GK_IC_FILE: int32 nvar (5), int32 nx, int32 ny, float64 cons[5][ny][nx]; config keys
 // gx, gy, gz enroll constant gravity; npoint and pg<q>_* enroll point masses; fixedbd = 1 fills
 // every ghost cell with the conserved state fix_0..fix_4, fixedbd = 2 with the plane read from
-// $GK_FIXED_FILE, and enrolls it as fixed-face data.  Lets
+// $GK_FIXED_FILE, and enrolls it as fixed-face data (fixinner = 1: with the inner region);
+// selfbd_<f> = 1 enrolls a self-defined face f (0 x left .. 3 y right) that writes the conserved
+// state sd<f>_0..sd<f>_4 into its ghost cells (with sdcopy_<f> = 1 it copies the edge cell
+// instead, which depends on the state); immersed = 1 sets up the immersed shell
+// (shell_x, shell_y, shell_z, shell_r).  Lets
 // any array (e.g. from gaukuk_amd.problems) start a run.
 #include <cstdio>
 #include <cstdlib>
@@ -40,6 +44,31 @@ static void ReadFixedPlane(TArray<Real>& cons, const Grid& grid) {
                     cons(v, k, j, i) = (Real)buf[((size_t)v * grid.leny + j) * grid.lenx + i];
 }
 
+// a self-defined face writing one conserved state into its ghost cells (active tangential range)
+static void ConstantFace(TArray<Real>& cons, const Grid& grid, int f) {
+    Real v[NVar];
+    for (int q = 0; q < NVar; q++)
+        v[q] = (Real)Config::getInstance().get("sd" + std::to_string(f) + "_" + std::to_string(q), 0.0);
+    const int il = f == 0 ? 0 : (f == 1 ? grid.ie : grid.ib);
+    const int ir = f == 0 ? grid.ib : (f == 1 ? grid.ie + grid.nGhost : grid.ie);
+    const int jl = f == 2 ? 0 : (f == 3 ? grid.je : grid.jb);
+    const int jr = f == 2 ? grid.jb : (f == 3 ? grid.je + grid.nGhost : grid.je);
+    const bool copy = (int)Config::getInstance().get("sdcopy_" + std::to_string(f), 0.0) == 1;
+    for (int q = 0; q < NVar; q++)
+        for (int k = grid.kb; k < grid.ke; k++)
+            for (int j = jl; j < jr; j++)
+                for (int i = il; i < ir; i++) {
+                    const int si = f == 0 ? grid.ib : (f == 1 ? grid.ie - 1 : i);
+                    const int sj = f == 2 ? grid.jb : (f == 3 ? grid.je - 1 : j);
+                    cons(q, k, j, i) = copy ? cons(q, k, sj, si) : v[q];
+                }
+}
+
+void Boundary::SelfDefineBDXL(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 0); }
+void Boundary::SelfDefineBDXR(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 1); }
+void Boundary::SelfDefineBDYL(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 2); }
+void Boundary::SelfDefineBDYR(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 3); }
+
 void Sim::Setup() {
     const char* path = std::getenv("GK_IC_FILE");
     if (!path) throw std::runtime_error("GK_IC_FILE is not set");
@@ -70,7 +99,15 @@ void Sim::Setup() {
         for (int j = 0; j < grid.ny; j++)
             for (int i = 0; i < grid.nx; i++)
                 cons(v, grid.kb, grid.jb + j, grid.ib + i) = (Real)buf[((size_t)v * grid.ny + j) * grid.nx + i];
-    if (fixedbd == 1 || fixedbd == 2) boundary.EnrollFixedBoundaryData(cons, grid);
+    const bool fixinner = (int)c.get("fixinner", 0.0) == 1;
+    if (fixedbd == 1 || fixedbd == 2 || fixinner) boundary.EnrollFixedBoundaryData(cons, grid, fixinner);
+    if ((int)c.get("selfbd_0", 0.0) == 1) boundary.EnrollSelfDefineBDXL();
+    if ((int)c.get("selfbd_1", 0.0) == 1) boundary.EnrollSelfDefineBDXR();
+    if ((int)c.get("selfbd_2", 0.0) == 1) boundary.EnrollSelfDefineBDYL();
+    if ((int)c.get("selfbd_3", 0.0) == 1) boundary.EnrollSelfDefineBDYR();
+    if ((int)c.get("immersed", 0.0) == 1)
+        boundary.SetupImmersedShell(grid, domain, (Real)c.get("shell_x", 0.0), (Real)c.get("shell_y", 0.0),
+                                    (Real)c.get("shell_z", 0.0), (Real)c.get("shell_r", 0.0));
     srcTerm.EnrollConstGravityVector(c.get("gx", 0.0), c.get("gy", 0.0), c.get("gz", 0.0));
     // npoint (1 or 2) and pg<q>_gm, _x, _y, _z, _vx, _vy, _vz, _rs enroll point masses in order
     const int npoint = (int)c.get("npoint", 0.0);

@@ -1,6 +1,7 @@
 // sim.cpp -- host driver: construction from the configuration, device set-up and Advance.
 #include "sim.hpp"
 
+#include <cstring>
 #include <iostream>
 #include <stdexcept>
 #include <vector>
@@ -8,6 +9,29 @@
 #include "utils/utils.hpp"
 
 namespace Gaukuk {
+
+namespace {
+
+// the ghost layers of face f (0 x left .. 5 z right) over the active tangential range
+void FaceGhostRange(const Grid& g, int f, int from[3], int to[3]) {
+    const int axis = f / 2;
+    const bool high = f % 2;
+    const int lo[3] = {g.ib, g.jb, g.kb}, hi[3] = {g.ie, g.je, g.ke};
+    for (int d = 0; d < 3; d++) { from[d] = lo[d]; to[d] = hi[d]; }
+    from[axis] = high ? hi[axis] : 0;
+    to[axis] = high ? hi[axis] + g.nGhost : lo[axis];
+}
+
+void CopyFace(const Grid& g, int f, const TArray<Real>& src, TArray<Real>& dst) {
+    int from[3], to[3];
+    FaceGhostRange(g, f, from, to);
+    for (int v = 0; v < NVar; v++)
+        for (int k = from[2]; k < to[2]; k++)
+            for (int j = from[1]; j < to[1]; j++)
+                for (int i = from[0]; i < to[0]; i++) dst(v, k, j, i) = src(v, k, j, i);
+}
+
+}  // namespace
 
 Sim::Sim() : isContinue(true), domain(grid), step(0), t(0), dt(1e10) {
     Config& c = Config::getInstance();
@@ -30,9 +54,34 @@ void Sim::Check(gk_status s, const char* what) const {
     if (s != GK_OK) throw std::runtime_error(std::string(what) + ": " + gk_last_error(ctx_));
 }
 
+// A self-defined face runs on the device as a fixed face when the ghost values it writes do not
+// depend on the state: it is applied to the current state and to one with every active value
+// changed, and the ghost layers it fills (the face's nGhost layers over the active tangential
+// range, which are all the stage reads) must agree bit for bit.  Returns the layers in `out`.
+void Sim::ProbeSelfDefined(int f, TArray<Real>& out) const {
+    static const char* names[6] = {"x left", "x right", "y left", "y right", "z left", "z right"};
+    TArray<Real> a = cons, b = cons;
+    for (int v = 0; v < NVar; v++)
+        for (int k = grid.kb; k < grid.ke; k++)
+            for (int j = grid.jb; j < grid.je; j++)
+                for (int i = grid.ib; i < grid.ie; i++) b(v, k, j, i) = a(v, k, j, i) * (Real)1.25 + (Real)0.5;
+    const Boundary::BoundaryFunc& fn = boundary.Face(f);
+    fn(a, grid, eos);
+    fn(b, grid, eos);
+    int from[3], to[3];
+    FaceGhostRange(grid, f, from, to);
+    for (int v = 0; v < NVar; v++)
+        for (int k = from[2]; k < to[2]; k++)
+            for (int j = from[1]; j < to[1]; j++)
+                for (int i = from[0]; i < to[0]; i++)
+                    if (std::memcmp(&a(v, k, j, i), &b(v, k, j, i), sizeof(Real)) != 0)
+                        throw std::runtime_error(std::string("the self-defined boundary on the ") + names[f] +
+                                                 " face depends on the state: only state-independent "
+                                                 "self-defined faces (e.g. inflows) run on the device");
+    out = a;
+}
+
 void Sim::EnsureDevice() {
-    if (boundary.AnySelfDefined())
-        throw std::runtime_error("self-defined boundaries have no device implementation in this build");
     if (!ctx_) {
         gk_grid g{grid.nx, grid.ny, grid.nz, grid.nGhost,
                   (double)domain.xmin, (double)domain.xmax, (double)domain.ymin,
@@ -47,6 +96,18 @@ void Sim::EnsureDevice() {
         p.rc_order = rcOrder == 1 ? 1 : 2;
         p.stepmax = stepMax;
         for (int f = 0; f < 6; f++) p.bc[f] = boundary.type[f];
+        // state-independent self-defined faces become fixed faces holding the ghost values their
+        // function writes
+        const int nface = grid.nz > 1 ? 6 : 4;
+        for (int f = 0; f < nface; f++) {
+            if (!boundary.selfDefined[f]) continue;
+            TArray<Real> ghosts;
+            ProbeSelfDefined(f, ghosts);
+            if (selfGhosts_.GetSize() == 0) selfGhosts_ = ghosts;
+            CopyFace(grid, f, ghosts, selfGhosts_);
+            selfFaces_[f] = true;
+            p.bc[f] = 4;
+        }
         p.precision = sizeof(Real) == 4 ? 32 : 64;
         gk_ctx* c = nullptr;
         gk_status s = gk_create(&g, &p, &c);
@@ -57,14 +118,33 @@ void Sim::EnsureDevice() {
             Check(gk_set_point_gravity(ctx_, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7]), "gk_set_point_gravity");
         }
     }
-    // fixed faces: the enrolled snapshot (re-sent when the setup enrolls new data)
-    if (boundary.AnyFixed() && boundary.FixedEnrolled() && fixedSent_ != boundary.FixedVersion()) {
-        Check(gk_set_fixed_boundary(ctx_, boundary.FixedData().data()), "gk_set_fixed_boundary");
+    // fixed faces and the inner region: the enrolled snapshot (re-sent when the setup enrolls new
+    // data), with the layers of state-independent self-defined faces written in
+    bool anySelf = false;
+    for (bool s : selfFaces_) anySelf |= s;
+    const bool wantFixed = anySelf || (boundary.FixedEnrolled() && (boundary.AnyFixed() || boundary.FixedInner()));
+    if (wantFixed && fixedSent_ != boundary.FixedVersion()) {
+        TArray<Real> snap = boundary.FixedEnrolled() ? boundary.FixedData() : cons;
+        for (int f = 0; f < 6; f++)
+            if (selfFaces_[f]) CopyFace(grid, f, selfGhosts_, snap);
+        Check(gk_set_fixed_boundary(ctx_, snap.data(), boundary.FixedEnrolled() && boundary.FixedInner() ? 1 : 0),
+              "gk_set_fixed_boundary");
         fixedSent_ = boundary.FixedVersion();
+    }
+    if (boundary.HasShell() && !shellSent_) {
+        const double* s = boundary.Shell();
+        Check(gk_set_immersed_shell(ctx_, s[0], s[1], s[2], s[3]), "gk_set_immersed_shell");
+        shellSent_ = true;
     }
     if (hostDirty_) {
         Check(gk_upload_cons(ctx_, cons.data()), "gk_upload_cons");
         hostDirty_ = false;
+    }
+    // the in-place part of the last boundary update (immersed shell, inner region)
+    if (boundary.DeviceUpdatePending()) {
+        Check(gk_update_bd(ctx_), "gk_update_bd");
+        boundary.ClearDeviceUpdate();
+        hostStale_ = true;
     }
 }
 

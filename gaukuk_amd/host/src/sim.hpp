@@ -62,7 +62,11 @@ private:
     gk_ctx* ctx_ = nullptr;
     bool hostDirty_ = true;   // host cons newer than the device copy
     bool hostStale_ = false;  // device newer than host cons
-    int fixedSent_ = 0;       // version of the fixed-face data the device holds
+    int fixedSent_ = -1;      // version of the fixed-face data the device holds (-1: none)
+    bool shellSent_ = false;  // the immersed shell is set up on the device
+    bool selfFaces_[6] = {false, false, false, false, false, false};  // self-defined faces run as fixed
+    TArray<Real> selfGhosts_;  // the ghost layers those faces write
+    void ProbeSelfDefined(int f, TArray<Real>& out) const;
     bool verbose_ = true;
     void EnsureDevice();
     void Check(gk_status s, const char* what) const;

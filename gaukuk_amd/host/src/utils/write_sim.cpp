@@ -12,6 +12,8 @@
 namespace Gaukuk {
 
 void Sim::WriteData(const int outputID, DataType dType) {
+    // a boundary update whose in-place part runs on the device is applied before writing
+    if (boundary.DeviceUpdatePending()) EnsureDevice();
     if (hostStale_) {
         Check(gk_download_cons(ctx_, cons.data()), "gk_download_cons");
         hostStale_ = false;

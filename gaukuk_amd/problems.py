@@ -59,6 +59,16 @@ class Problem:
     # (Boundary::EnrollFixedBoundaryData, src/boundary/fixed.cpp:242-269): one conserved state for
     # every ghost cell (5 values) or a ghost-inclusive array [5][NY+2g][NX+2g]
     fixed: tuple = ()
+    # self-defined faces (Boundary::EnrollSelfDefineBD*, src/boundary/boundary.hpp:28-45) of the
+    # state-independent kind: (face, conserved state) pairs, face 0..3 = xleft, xright, yleft,
+    # yright, whose function writes that state into the face's ghost cells (the inflow of
+    # src/setup/setup_solid_body.cpp)
+    self_faces: tuple = ()
+    # EnrollFixedBoundaryData(..., fixInner = true): cells of the inner region are reset to the
+    # enrolled state before every stage (src/boundary/fixed.cpp:181-207)
+    fix_inner: bool = False
+    # immersed reflective sphere (x, y, z, r) (Boundary::SetupImmersedShell, src/boundary/immersed.cpp)
+    shell: tuple = ()
     ng: int = 2
     cons: np.ndarray = field(default=None, repr=False)  # (5, ny, nx) float64
 
@@ -77,8 +87,48 @@ class Problem:
             raise ValueError(f"fixed-face data has shape {f.shape}, expected {shape}")
         return f.copy()
 
+    def device_bc(self) -> tuple:
+        """Face types as the device runs them: a state-independent self-defined face is a fixed face."""
+        bc = list(self.bc)
+        for f, _ in self.self_faces:
+            bc[f] = 4
+        return tuple(bc)
+
+    def enrolled_state(self):
+        """The ghost-inclusive state [5][NY+2g][NX+2g] the setup holds when it enrolls its fixed data
+        (active cells = cons; ghosts of type-4 faces from `fixed`; ghosts of self-defined faces hold
+        their state), or None when nothing is enrolled."""
+        if not (len(self.fixed) or self.self_faces or self.fix_inner):
+            return None
+        g = self.ng
+        full = self.fixed_full()
+        if full is None:
+            full = np.zeros((5, self.ny + 2 * g, self.nx + 2 * g))
+        for f, state in self.self_faces:
+            st = np.asarray(state, np.float64)[:, None, None]
+            if f == 0:
+                full[:, g:g + self.ny, :g] = st
+            elif f == 1:
+                full[:, g:g + self.ny, g + self.nx:] = st
+            elif f == 2:
+                full[:, :g, g:g + self.nx] = st
+            else:
+                full[:, g + self.ny:, g:g + self.nx] = st
+        full[:, g:g + self.ny, g:g + self.nx] = self.cons
+        return full
+
     def _point_keys(self) -> dict:
         keys = {}
+        for f, state in self.self_faces:
+            keys[f"selfbd_{f}"] = 1
+            for q, v in enumerate(state):
+                keys[f"sd{f}_{q}"] = v
+        if self.fix_inner:
+            keys["fixinner"] = 1
+        if self.shell:
+            keys["immersed"] = 1
+            for name, v in zip(("x", "y", "z", "r"), self.shell):
+                keys[f"shell_{name}"] = v
         if len(self.fixed):
             if np.ndim(self.fixed) == 1:
                 keys["fixedbd"] = 1
@@ -200,6 +250,25 @@ def blast(nx: int, ny: int, p_ratio: float = 50.0, radius: float = 0.1, center=N
     base = dict(name="blast", nx=nx, ny=ny, extent=ext, integrator=2, rc_order=2, cfl=0.4, bc=bcs)
     base.update(kw)
     return Problem(cons=prim_to_cons(rho, z, z, z, p, base.get("gamma", 1.4)), **base)
+
+
+def solid_body(nx: int = 1024, ny: int | None = None, rho: float = 1.0, p: float = 1.0, vx_in: float = 4.0,
+               x0: float = -3.0, y0: float = 0.0, z0: float = 0.0, r_solid: float = 1.0,
+               extent=(-5.0, 5.0, -5.0, 5.0, -0.5, 0.5), **kw) -> Problem:
+    """Supersonic flow past a reflective cylinder, as src/setup/setup_solid_body.cpp with
+    input/solid_body.in: uniform (rho, vx_in, p), an x-left face that injects that state (the
+    setup's self-defined face), outflow elsewhere, an immersed shell of radius r_solid at (x0, y0, z0);
+    RK2, CFL 0.5."""
+    ny = nx if ny is None else ny
+    x, _ = _centres(extent, nx, ny, 2)
+    one = np.ones_like(x)
+    gamma = kw.get("gamma", 1.4)
+    cons = prim_to_cons(rho * one, vx_in * one, 0 * one, 0 * one, p * one, gamma)
+    inflow = (rho, rho * vx_in, 0.0, 0.0, 0.5 * rho * (vx_in * vx_in) + p * (1.0 / (gamma - 1.0)))
+    base = dict(name="solid_body", nx=nx, ny=ny, extent=tuple(extent), integrator=2, rc_order=2, cfl=0.5,
+                bc=(3, 3, 3, 3), self_faces=((0, inflow),), shell=(x0, y0, z0, r_solid))
+    base.update(kw)
+    return Problem(cons=cons, **base)
 
 
 def riemann_1d(nx: int, left, right, x0: float = 0.5, **kw) -> Problem:

@@ -66,11 +66,27 @@ typedef struct {
 gk_status gk_set_point_gravity(gk_ctx* ctx, double gm, double x, double y, double z, double vx, double vy, double vz,
                                double rs);
 
-/* Boundary::EnrollFixedBoundaryData (src/boundary/fixed.cpp:242-269; faces FixedBDXL..FixedBDYR, fixed.cpp:16-123):
- * `cons` is a state in the reference layout and the context's precision; its ghost layers become the
- * values that type-4 faces restore before every stage.  Advancing a context with a type-4 face
- * fails with GK_ESTATE until this is called, as the reference throws. */
-gk_status gk_set_fixed_boundary(gk_ctx* ctx, const void* cons);
+/* Boundary::EnrollFixedBoundaryData(cons, grid, fixInner) (src/boundary/fixed.cpp:242-269; faces
+ * FixedBDXL..FixedBDYR, fixed.cpp:16-123): `cons` is a state in the reference layout and the
+ * context's precision; its ghost layers become the values that type-4 faces restore before every
+ * stage.  With fix_inner != 0 its active cells inside the inner region (the central half-box, cell
+ * centres within 0.5 of the origin: Boundary::FixedBDInner, fixed.cpp:181-207) are restored
+ * there before every stage as well.  Advancing a context with a type-4 face fails with GK_ESTATE
+ * until this is called, as the reference throws. */
+gk_status gk_set_fixed_boundary(gk_ctx* ctx, const void* cons, int fix_inner);
+
+/* Boundary::SetupImmersedShell (src/boundary/immersed.cpp:89-235): a reflective solid sphere of
+ * radius r at (x, y, z).  Before every stage the shell cells (inside the body, within nGhost cells
+ * of its surface) take the weighted image of the fluid around their mirror points
+ * (Boundary::ImmersedReflective, immersed.cpp:237-300).  A later call replaces the shell.  Fails
+ * with GK_EINVAL where the reference throws (body outside the domain, too close to a face, more
+ * than 8 image cells); advancing with gravity enrolled fails with GK_EINVAL. */
+gk_status gk_set_immersed_shell(gk_ctx* ctx, double x, double y, double z, double r);
+
+/* The in-place part of Boundary::UpdateBD on the current state (src/boundary/boundary.cpp:127-142):
+ * the immersed shell, then the inner fixed region.  The reference's main calls UpdateBD once after
+ * Setup (src/main.cpp:30); callers that upload a state and advance it call this in between. */
+gk_status gk_update_bd(gk_ctx* ctx);
 
 /* Sim::Sim() (src/sim.cpp:45-67): validate, allocate device state, select the integrator. */
 gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out);

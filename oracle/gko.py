@@ -43,8 +43,11 @@ def lib(flavour: str = "strict") -> C.CDLL:
         L.gko_step.restype = ip; L.gko_step.argtypes = [vp]
         L.gko_is_continue.restype = ip; L.gko_is_continue.argtypes = [vp]
         L.gko_last_dt.restype = C.c_double; L.gko_last_dt.argtypes = [vp]
+        L.gko_update_bd.restype = ip
         L.gko_update_bd.argtypes = [vp]
-        L.gko_enroll_fixed.argtypes = [vp]
+        L.gko_enroll_fixed.argtypes = [vp, ip]
+        L.gko_setup_immersed.restype = ip
+        L.gko_setup_immersed.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.gko_dt_log.restype = ip; L.gko_dt_log.argtypes = [vp, dp, dp, ip]
         L.gko_hllc.argtypes = [ip, dp, dp, ip, C.c_double, dp]
         L.gko_plm_row.argtypes = [ip, dp, dp, dp]
@@ -66,7 +69,7 @@ def make_params(prob) -> Params:
     p.gamma, p.cfl = prob.gamma, prob.cfl
     p.integrator, p.rc_order, p.stepmax = prob.integrator, prob.rc_order, prob.stepmax
     for k in range(4):
-        p.bc[k] = prob.bc[k]
+        p.bc[k] = prob.device_bc()[k]
     for k in range(3):
         p.g[k] = prob.g[k]
     p.npoint = len(prob.point)
@@ -90,14 +93,17 @@ class OracleSim:
         ng = prob.ng
         self.shape = (5, prob.ny + 2 * ng, prob.nx + 2 * ng)
         full = np.zeros(self.shape, self.dtype)
-        fixed = prob.fixed_full()
-        if fixed is not None:
+        enrolled = prob.enrolled_state()
+        if enrolled is not None:
             # a setup that fills the ghost cells, then enrolls the fixed data
-            full[:] = fixed
+            full[:] = enrolled
         full[:, ng:ng + prob.ny, ng:ng + prob.nx] = prob.cons
         self.L.gko_set_cons(self.h, full.ctypes.data)
-        if fixed is not None:
-            self.L.gko_enroll_fixed(self.h)
+        if enrolled is not None:
+            self.L.gko_enroll_fixed(self.h, int(bool(prob.fix_inner)))
+        if prob.shell:
+            if self.L.gko_setup_immersed(self.h, *[float(v) for v in prob.shell]) != 0:
+                raise ValueError("the oracle rejected the immersed shell (out of domain or too close to a face)")
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -112,7 +118,7 @@ class OracleSim:
         snap = np.ascontiguousarray(full, self.dtype)
         assert snap.shape == self.shape
         self.L.gko_set_cons(self.h, snap.ctypes.data)
-        self.L.gko_enroll_fixed(self.h)
+        self.L.gko_enroll_fixed(self.h, 0)
         self.L.gko_set_cons(self.h, cur.ctypes.data)
 
     def full(self) -> np.ndarray:
@@ -124,7 +130,16 @@ class OracleSim:
         ng = self.prob.ng
         return self.full()[:, ng:ng + self.prob.ny, ng:ng + self.prob.nx].copy()
 
+    def update_bd(self):
+        """Boundary::UpdateBD on the current state, as main does once after Setup (src/main.cpp:30):
+        ghost cells, and in place the immersed shell and the inner fixed region."""
+        if self.L.gko_update_bd(self.h) != 0:
+            raise ValueError("boundary update failed (fixed data not enrolled?)")
+        self._bd_done = True
+
     def advance(self, dtoutput: float) -> int:
+        if not getattr(self, "_bd_done", False):
+            self.update_bd()
         return self.L.gko_advance(self.h, dtoutput)
 
     @property

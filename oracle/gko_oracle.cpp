@@ -211,7 +211,11 @@ struct Oracle {
     // EnrollFixedBoundaryData (fixed.cpp:242-269) are copied back over the active tangential
     // range.  Here the saved layers are a snapshot of the whole ghosted array.
     std::vector<Real> fixedSnap;
-    void EnrollFixedBoundaryData() { fixedSnap = cons; }
+    std::vector<Real> fixedInner;   // EnrollFixedBoundaryData(..., fixInner = true): consFixedInner = cons
+    void EnrollFixedBoundaryData(bool fixInner = false) {
+        fixedSnap = cons;
+        if (fixInner) fixedInner = cons;
+    }
     void FillFixedX(std::vector<Real>& U, bool left) {
         if (fixedSnap.empty())
             throw std::invalid_argument("fixed boundary is not initialized");
@@ -228,11 +232,173 @@ struct Oracle {
             for (int j = gl; j < gr; j++)
                 for (int i = ib; i < ie; i++) U[at(n, j, i)] = fixedSnap[at(n, j, i)];
     }
+    // Inner fixed region: src/boundary/fixed.cpp:181-207.  Cells of the central half-box whose
+    // centre lies within 0.5 of the origin take the saved state.
+    void FixedBDInner(std::vector<Real>& U) {
+        const int il = static_cast<int>(ib + 0.25 * (ie - ib));
+        const int ir = static_cast<int>(ib + 0.75 * (ie - ib));
+        const int jl = static_cast<int>(jb + 0.25 * (je - jb));
+        const int jr = static_cast<int>(jb + 0.75 * (je - jb));
+        for (int n = 0; n < NV; n++)
+            for (int j = jl; j < jr; j++)
+                for (int i = il; i < ir; i++) {
+                    const Real xnow = xc[i], ynow = yc[j], znow = zc0;
+                    const Real r2 = xnow * xnow + ynow * ynow + znow * znow;
+                    if (r2 < 0.5 * 0.5) U[at(n, j, i)] = fixedInner[at(n, j, i)];
+                }
+    }
+
+    // ---- Immersed reflective shell: src/boundary/immersed.cpp -----------------------------
+    static constexpr int nTarMax = 8;
+    struct Shell {
+        int i, j, nTar;
+        int ti[nTarMax + 1], tj[nTarMax + 1];
+        Real cosp, sinp, cost, sint;
+        Real gam[nTarMax + 1];
+    };
+    std::vector<Shell> shells;
+    bool immersed = false;
+    // FindLeftId / FindRightId: immersed.cpp:19-47
+    static int FindLeftId(const std::vector<Real>& c, int b, int e, Real x0, Real r, Real lo) {
+        const Real xb = x0 - 1.2 * r;
+        if (xb < lo) throw std::invalid_argument("immersed boundary out of domain");
+        for (int i = b; i < e; i++)
+            if (c[i] >= xb) return i;
+        return e;
+    }
+    static int FindRightId(const std::vector<Real>& c, int b, int e, Real x0, Real r, Real hi) {
+        const Real xe = x0 + 1.2 * r;
+        if (xe > hi) throw std::invalid_argument("immersed boundary out of domain");
+        for (int i = e - 1; i >= b; i--)
+            if (c[i] <= xe) return i;
+        return b;
+    }
+    // IsShellCell: immersed.cpp:49-69 -- inside the body, with a neighbour nGhost cells away outside
+    static bool IsShellCell(Real dx_, Real dy_, Real dz_, Real r, Real sx, Real sy, Real sz) {
+        Real rn = std::sqrt(dx_ * dx_ + dy_ * dy_ + dz_ * dz_);
+        if (rn < r) {
+            rn = std::sqrt((dx_ + sx) * (dx_ + sx) + dy_ * dy_ + dz_ * dz_); if (rn > r) return true;
+            rn = std::sqrt((dx_ - sx) * (dx_ - sx) + dy_ * dy_ + dz_ * dz_); if (rn > r) return true;
+            rn = std::sqrt(dx_ * dx_ + (dy_ + sy) * (dy_ + sy) + dz_ * dz_); if (rn > r) return true;
+            rn = std::sqrt(dx_ * dx_ + (dy_ - sy) * (dy_ - sy) + dz_ * dz_); if (rn > r) return true;
+            rn = std::sqrt(dx_ * dx_ + dy_ * dy_ + (dz_ + sz) * (dz_ + sz)); if (rn > r) return true;
+            rn = std::sqrt(dx_ * dx_ + dy_ * dy_ + (dz_ - sz) * (dz_ - sz)); if (rn > r) return true;
+        }
+        return false;
+    }
+    // Boundary::SetupImmersedShell: immersed.cpp:89-235 (nz == 1: the active z plane only).  The
+    // shell cells are listed in (j, i) order; the reference's order depends on its threads, and the
+    // result does not (shell cells and their targets are disjoint).
+    void SetupImmersedShell(Real xs, Real ys, Real zs, Real rs) {
+        if (nx <= 1 || ny <= 1) throw std::invalid_argument("immersed boundary has to be at least 2D. ");
+        immersed = true;
+        const int il = FindLeftId(xc, ib, ie, xs, rs, (Real)P.xmin);
+        const int ir = FindRightId(xc, ib, ie, xs, rs, (Real)P.xmax);
+        const int jl = FindLeftId(yc, jb, je, ys, rs, (Real)P.ymin);
+        const int jr = FindRightId(yc, jb, je, ys, rs, (Real)P.ymax);
+        if (il > ir || jl > jr) throw std::invalid_argument("immersed boundary out of domain");
+        shells.clear();
+        const Real sx = ng * dx, sy = ng * dy, sz = ng * dz;
+        for (int j = jl; j < jr; j++)
+            for (int i = il; i < ir; i++)
+                if (IsShellCell(xc[i] - xs, yc[j] - ys, zc0 - zs, rs, sx, sy, sz)) {
+                    Shell s{};
+                    s.i = i; s.j = j;
+                    shells.push_back(s);
+                }
+        const int nxs = 2 * ng, nys = 2 * ng;
+        for (Shell& s : shells) {
+            s.ti[0] = s.i; s.tj[0] = s.j;
+            const Real delx = xc[s.i] - xs, dely = yc[s.j] - ys, delz = zc0 - zs;
+            const Real rnow = std::sqrt(delx * delx + dely * dely + delz * delz);
+            Real cosp, sinp, cost, sint;
+            const Real rxy = std::sqrt(delx * delx + dely * dely);
+            if (rxy < 1e-15) { cosp = 1.0; sinp = 0.0; } else { cosp = delx / rxy; sinp = dely / rxy; }
+            if (rnow < 1e-15) { cost = 1.0; sint = 0.0; } else { cost = delz / rnow; sint = rxy / rnow; }
+            s.cosp = cosp; s.sinp = sinp; s.cost = cost; s.sint = sint;
+            const Real xImg = sint * cosp * (2 * rs - rnow) + xs;
+            const Real yImg = sint * sinp * (2 * rs - rnow) + ys;
+            const Real zImg = cost * (2 * rs - rnow) + zs;
+            int nt = 0;
+            Real disDel[nTarMax + 1];
+            disDel[0] = 2 * (rs - rnow);
+            for (int ii = -nxs; ii < nxs + 1; ++ii)
+                for (int jj = -nys; jj < nys + 1; ++jj) {
+                    const int iT = s.i + ii, jT = s.j + jj;
+                    if (iT < ib || iT >= ie || jT < jb || jT >= je)
+                        throw std::invalid_argument("immersed body is too close to the domain boundary, "
+                                                    "neighboor search out of range.");
+                    const Real xT = xc[iT], yT = yc[jT], zT = zc0;
+                    if (std::abs(xT - xImg) <= dx && std::abs(yT - yImg) <= dy && std::abs(zT - zImg) <= dz) {
+                        const Real rT = std::sqrt((xT - xs) * (xT - xs) + (yT - ys) * (yT - ys) + (zT - zs) * (zT - zs));
+                        if (rT > rs) {
+                            nt++;
+                            if (nt > nTarMax) throw std::logic_error("n of imagine points is more than nTarMax");
+                            s.ti[nt] = iT; s.tj[nt] = jT;
+                            disDel[nt] = std::sqrt((xT - xImg) * (xT - xImg) + (yT - yImg) * (yT - yImg) +
+                                                   (zT - zImg) * (zT - zImg));
+                        }
+                    }
+                }
+            s.nTar = nt;
+            Real sum = 0.;
+            for (int n = 0; n < nt + 1; n++) sum = sum + 1. / (disDel[n] * disDel[n]);
+            for (int n = 0; n < nt + 1; n++) s.gam[n] = 1. / (disDel[n] * disDel[n] * sum);
+        }
+    }
+    // Boundary::ImmersedReflective: immersed.cpp:237-300 -- each shell cell takes the weighted image
+    // of its targets with the radial velocity reflected
+    void ImmersedReflective(std::vector<Real>& U) {
+        const Real dmin = (Real)1e-16, pmin = (Real)1e-16;
+        const Real gm1 = gamma - 1;
+        for (const Shell& s : shells) {
+            const Real cosp = s.cosp, sinp = s.sinp, cost = s.cost, sint = s.sint;
+            Real dens = 0, vr = 0, vp = 0, vt = 0, pres = 0;
+            const Real g0 = s.gam[0];
+            for (int n = 1; n <= s.nTar; n++) {
+                const int iT = s.ti[n], jT = s.tj[n];
+                Real denTar = U[at(DEN, jT, iT)];
+                denTar = std::max(denTar, dmin);
+                const Real denInv = 1.0 / denTar;
+                dens += 1. / (1 - g0) * s.gam[n] * denTar;
+                const Real vx = U[at(MX, jT, iT)] * denInv;
+                const Real vy = U[at(MY, jT, iT)] * denInv;
+                const Real vz = U[at(MZ, jT, iT)] * denInv;
+                // CarToLocal: immersed.cpp:73-79
+                const Real vrl = vx * cosp * sint + vy * sinp * sint + vz * cost;
+                const Real vpl = -vx * sinp + vy * cosp;
+                const Real vtl = vx * cosp * cost + vy * sinp * cost - vz * sint;
+                vr += 1. / (-1. - g0) * s.gam[n] * vrl;
+                vp += 1. / (1. - g0) * s.gam[n] * vpl;
+                vt += 1. / (1. - g0) * s.gam[n] * vtl;
+                const Real engTar = U[at(ENG, jT, iT)];
+                const Real engKin = 0.5 * denTar * (vx * vx + vy * vy + vz * vz);
+                Real preTar = gm1 * (engTar - engKin);
+                preTar = (preTar > pmin) ? preTar : pmin;
+                pres += 1. / (1. - g0) * s.gam[n] * preTar;
+            }
+            // LocalToCar: immersed.cpp:82-87
+            const Real vx = vr * sint * cosp + vt * cost * cosp - vp * sinp;
+            const Real vy = vr * sint * sinp + vt * cost * sinp + vp * cosp;
+            const Real vz = vr * cost - vt * sint;
+            U[at(DEN, s.j, s.i)] = dens;
+            U[at(MX, s.j, s.i)] = dens * vx;
+            U[at(MY, s.j, s.i)] = dens * vy;
+            U[at(MZ, s.j, s.i)] = dens * vz;
+            const Real engKin = 0.5 * dens * (vx * vx + vy * vy + vz * vz);
+            U[at(ENG, s.j, s.i)] = engKin + pres * gm1Rec;   // EGas: eos.hpp:30-32
+        }
+    }
+
+    // Boundary::UpdateBD: src/boundary/boundary.cpp:127-142 -- faces, then the immersed shell,
+    // then the inner fixed region
     void UpdateBD(std::vector<Real>& U) {
         if (P.bc[0] == 4) FillFixedX(U, true); else FillX(U, true, P.bc[0]);
         if (P.bc[1] == 4) FillFixedX(U, false); else FillX(U, false, P.bc[1]);
         if (P.bc[2] == 4) FillFixedY(U, true); else FillY(U, true, P.bc[2]);
         if (P.bc[3] == 4) FillFixedY(U, false); else FillY(U, false, P.bc[3]);
+        if (immersed) ImmersedReflective(U);
+        if (!fixedInner.empty()) FixedBDInner(U);
     }
 
     // ---- Reconstruction + HLLC ---------------------------------------------------------
@@ -568,11 +734,26 @@ double gko_time(const gko_sim* s) { return s->h.f ? (double)s->h.f->t : s->h.d->
 int gko_step(const gko_sim* s) { return s->h.f ? s->h.f->step : s->h.d->step; }
 int gko_is_continue(const gko_sim* s) { return s->h.f ? s->h.f->isContinue : s->h.d->isContinue; }
 double gko_last_dt(const gko_sim* s) { return s->h.f ? (double)s->h.f->dt : s->h.d->dt; }
-void gko_enroll_fixed(gko_sim* s) {
-    if (s->h.f) s->h.f->EnrollFixedBoundaryData(); else s->h.d->EnrollFixedBoundaryData();
+void gko_enroll_fixed(gko_sim* s, int fix_inner) {
+    if (s->h.f) s->h.f->EnrollFixedBoundaryData(fix_inner != 0);
+    else s->h.d->EnrollFixedBoundaryData(fix_inner != 0);
 }
-void gko_update_bd(gko_sim* s) {
-    if (s->h.f) s->h.f->UpdateBD(s->h.f->cons); else s->h.d->UpdateBD(s->h.d->cons);
+int gko_setup_immersed(gko_sim* s, double x, double y, double z, double r) {
+    try {
+        if (s->h.f) s->h.f->SetupImmersedShell((float)x, (float)y, (float)z, (float)r);
+        else s->h.d->SetupImmersedShell(x, y, z, r);
+    } catch (...) {
+        return -1;
+    }
+    return 0;
+}
+int gko_update_bd(gko_sim* s) {
+    try {
+        if (s->h.f) s->h.f->UpdateBD(s->h.f->cons); else s->h.d->UpdateBD(s->h.d->cons);
+    } catch (...) {
+        return -1;
+    }
+    return 0;
 }
 int gko_dt_log(const gko_sim* s, double* t_out, double* dt_out, int cap) {
     const std::vector<double>& lt = s->h.f ? s->h.f->log_t : s->h.d->log_t;

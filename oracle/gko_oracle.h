@@ -18,6 +18,8 @@
  *   EquationOfState              src/eos/adiabatic.cpp:12-198, eos.hpp:27-32
  *   Boundary types 0..3          src/boundary/{outflow_copy,periodic,reflective,outflow}.cpp
  *   Boundary fixed (type 4)      src/boundary/fixed.cpp:16-127, 242-269
+ *   Boundary inner fixed region  src/boundary/fixed.cpp:181-207
+ *   Immersed reflective shell    src/boundary/immersed.cpp:19-300
  *   SourceTerm::ConstGravity     src/source_term/const_gravity.cpp:10-67
  *   SourceTerm::PointGravity     src/source_term/point_gravity.cpp:11-93
  *   SourceTerm::BinaryGravity    src/source_term/binary_gravity.cpp:11-101
@@ -67,10 +69,15 @@ int      gko_step(const gko_sim* s);
 int      gko_is_continue(const gko_sim* s);
 double   gko_last_dt(const gko_sim* s);
 /* Boundary::EnrollFixedBoundaryData (src/boundary/fixed.cpp:242-269): save the ghost layers of
- * the current (set) state for type-4 faces. */
-void     gko_enroll_fixed(gko_sim* s);
-/* Boundary::UpdateBD on the current state (used before output). */
-void     gko_update_bd(gko_sim* s);
+ * the current (set) state for type-4 faces, and with fix_inner the state of the inner region
+ * (fixed.cpp:181-207). */
+void     gko_enroll_fixed(gko_sim* s, int fix_inner);
+/* Boundary::SetupImmersedShell (src/boundary/immersed.cpp:89-235): a reflective solid sphere of
+ * radius r at (x, y, z), applied at every boundary update.  Returns -1 where the reference throws. */
+int      gko_setup_immersed(gko_sim* s, double x, double y, double z, double r);
+/* Boundary::UpdateBD on the current state (main calls it once after Setup, src/main.cpp:30).
+ * Returns -1 where the reference throws. */
+int      gko_update_bd(gko_sim* s);
 /* Current step log: up to cap entries of (t, dt) for the last gko_advance. */
 int      gko_dt_log(const gko_sim* s, double* t_out, double* dt_out, int cap);
 

@@ -12,7 +12,12 @@
 // (SourceTerm::EnrollPointGravity, src/source_term/source.cpp:33-67; two masses = binary gravity);
 // fixedbd = 1 fills every ghost cell with the conserved state fix_0..fix_4, fixedbd = 2 with the
 // plane read from $GK_FIXED_FILE, and enrolls the fixed boundary data
-// (Boundary::EnrollFixedBoundaryData, src/boundary/fixed.cpp:242-269).
+// (Boundary::EnrollFixedBoundaryData, src/boundary/fixed.cpp:242-269); fixinner = 1 enrolls it with
+// the inner region (fixed.cpp:181-207); selfbd_<f> = 1 enrolls a self-defined face f (0 xleft,
+// 1 xright, 2 yleft, 3 yright) whose ghost cells take the conserved state sd<f>_0..sd<f>_4 (the
+// form of the inflow face of src/setup/setup_solid_body.cpp); immersed = 1 sets up the immersed
+// shell at shell_x, shell_y, shell_z with radius shell_r (Boundary::SetupImmersedShell,
+// src/boundary/immersed.cpp:89-235).
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -48,6 +53,26 @@ static void ReadFixedPlane(TArray<Real>& cons, const Grid& grid) {
                     cons(v, k, j, i) = (Real)buf[((size_t)v * grid.leny + j) * grid.lenx + i];
 }
 
+// a self-defined face writing one conserved state into its ghost cells (active tangential range)
+static void ConstantFace(TArray<Real>& cons, const Grid& grid, int f) {
+    Real v[NVar];
+    for (int q = 0; q < NVar; q++)
+        v[q] = (Real)Config::getInstance().get("sd" + std::to_string(f) + "_" + std::to_string(q), 0.0);
+    const int il = f == 0 ? grid.igb : (f == 1 ? grid.ie : grid.ib);
+    const int ir = f == 0 ? grid.ib : (f == 1 ? grid.ige : grid.ie);
+    const int jl = f == 2 ? grid.jgb : (f == 3 ? grid.je : grid.jb);
+    const int jr = f == 2 ? grid.jb : (f == 3 ? grid.jge : grid.je);
+    for (int q = 0; q < NVar; q++)
+        for (int k = grid.kb; k < grid.ke; k++)
+            for (int j = jl; j < jr; j++)
+                for (int i = il; i < ir; i++) cons(q, k, j, i) = v[q];
+}
+
+void Boundary::SelfDefineBDXL(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 0); }
+void Boundary::SelfDefineBDXR(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 1); }
+void Boundary::SelfDefineBDYL(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 2); }
+void Boundary::SelfDefineBDYR(TArray<Real>& cons, const Grid& grid, const EquationOfState&) { ConstantFace(cons, grid, 3); }
+
 void Sim::Setup() {
     const char* path = std::getenv("GK_IC_FILE");
     if (!path) throw std::runtime_error("GK_IC_FILE is not set");
@@ -80,7 +105,16 @@ void Sim::Setup() {
     Real gy = Config::getInstance().get("gy", 0.0);
     Real gz = Config::getInstance().get("gz", 0.0);
     srcTerm.EnrollConstGravityVector(gx, gy, gz);
-    if (fixedbd == 1 || fixedbd == 2) boundary.EnrollFixedBoundaryData(cons, grid);
+    Config& c = Config::getInstance();
+    const bool fixinner = (int)c.get("fixinner", 0.0) == 1;
+    if (fixedbd == 1 || fixedbd == 2 || fixinner) boundary.EnrollFixedBoundaryData(cons, grid, fixinner);
+    if ((int)c.get("selfbd_0", 0.0) == 1) boundary.EnrollSelfDefineBDXL();
+    if ((int)c.get("selfbd_1", 0.0) == 1) boundary.EnrollSelfDefineBDXR();
+    if ((int)c.get("selfbd_2", 0.0) == 1) boundary.EnrollSelfDefineBDYL();
+    if ((int)c.get("selfbd_3", 0.0) == 1) boundary.EnrollSelfDefineBDYR();
+    if ((int)c.get("immersed", 0.0) == 1)
+        boundary.SetupImmersedShell(grid, domain, (Real)c.get("shell_x", 0.0), (Real)c.get("shell_y", 0.0),
+                                    (Real)c.get("shell_z", 0.0), (Real)c.get("shell_r", 0.0));
     const int npoint = (int)Config::getInstance().get("npoint", 0.0);
     for (int q = 0; q < npoint; q++) {
         auto key = [&](const char* name) { return "pg" + std::to_string(q) + "_" + name; };

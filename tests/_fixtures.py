@@ -26,7 +26,11 @@ def load_run(name):
         g=(m["gx"], m["gy"], m["gz"]), ng=int(m["NGhost"]), cons=z["ic"].copy(),
         point=tuple(tuple(m[f"pg{q}_{k}"] for k in ("gm", "x", "y", "z", "vx", "vy", "vz", "rs"))
                     for q in range(int(m.get("npoint", 0)))),
-        fixed=_fixed(z, m))
+        fixed=_fixed(z, m),
+        self_faces=tuple((f, tuple(m[f"sd{f}_{q}"] for q in range(5))) for f in range(4)
+                         if int(m.get(f"selfbd_{f}", 0))),
+        fix_inner=bool(int(m.get("fixinner", 0))),
+        shell=tuple(m[f"shell_{k}"] for k in ("x", "y", "z", "r")) if int(m.get("immersed", 0)) else ())
     return prob, z["final"].copy(), float(z["t"]), int(z["steps"])
 
 

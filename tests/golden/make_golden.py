@@ -77,7 +77,30 @@ def cases():
     fsnap = P.prim_to_cons(rng.uniform(0.8, 1.2, shp), rng.uniform(-0.3, 0.3, shp), rng.uniform(-0.3, 0.3, shp),
                            np.zeros(shp), rng.uniform(0.2, 0.4, shp), fv.gamma)
     out["fixedvar52x44_rk2_g"] = fv.with_(integrator=2, g=(0.0, -0.1, 0.0), fixed=fsnap)
+    # the reference's solid-body setup at 96^2: self-defined inflow face + immersed cylinder
+    out["solid96_rk2"] = P.solid_body(96, stepmax=40)
+    # an inner fixed region (a blast that overlaps it) under gravity, RK3
+    inner = P.blast(64, 64, p_ratio=30.0, radius=0.3, center=(0.25, 0.15), bc=(1, 1, 2, 2), stepmax=30)
+    out["inner64_rk3_g"] = inner.with_(extent=(-1.0, 1.0, -1.0, 1.0, -0.5, 0.5), integrator=3, g=(0.0, -0.1, 0.0),
+                                       fix_inner=True, cons=_shifted_blast(inner))
+    # an immersed shell overlapping the inner fixed region, oblique flow, periodic x, reflective y
+    sh = P.solid_body(72, 60, vx_in=1.5, x0=0.4, y0=-0.3, r_solid=0.8, extent=(-3.0, 3.0, -2.5, 2.5, -0.5, 0.5),
+                      integrator=3, stepmax=30, bc=(1, 1, 2, 2), self_faces=(), fix_inner=True)
+    rho, mx, my, mz, e = sh.cons
+    sh.cons = P.prim_to_cons(rho, mx / rho, 0.3 + 0 * rho, 0 * rho, 0.8 + 0 * rho, sh.gamma)
+    out["shellinner72x60_rk3"] = sh
     return out
+
+
+def _shifted_blast(prob):
+    """The blast state of `prob` re-centred for the [-1, 1]^2 domain (pressure jump at r = 0.3
+    around (0.25, 0.15) in the new coordinates)."""
+    x = -1.0 + (np.arange(prob.nx) + 0.5) * 2.0 / prob.nx
+    y = -1.0 + (np.arange(prob.ny) + 0.5) * 2.0 / prob.ny
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    p = np.where((X - 0.25) ** 2 + (Y - 0.15) ** 2 < 0.09, 3.0, 0.1)
+    z = np.zeros_like(X)
+    return P.prim_to_cons(1.0 + 0.2 * X, z, z, z, p, prob.gamma)
 
 
 def run_reference(prob, workdir):

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from gaukuk_amd.gkio import GaukukOutput
+from gaukuk_amd.config import write_in
 from tests._driver import HOST, driver, run_from_file
 from tests._fixtures import load_run
 
@@ -25,7 +26,8 @@ def rel_l1(a, b):
 
 @pytest.mark.parametrize("name", ["kh64_rk3", "rt32x96_rk2_g", "blast48x56_mixed", "sod128_fe_rc1",
                                   "point48_rk2_bc3", "binary40x44_rk3_g", "fixed48x40_rk3",
-                                  "fixedvar52x44_rk2_g"])
+                                  "fixedvar52x44_rk2_g", "solid96_rk2", "inner64_rk3_g",
+                                  "shellinner72x60_rk3"])
 def test_driver_golden(tmp_path, name):
     prob, final, t, steps = load_run(name)
     print(f"driver case {name}", flush=True)
@@ -52,3 +54,44 @@ def test_driver_sod_setup_matches_reference(tmp_path):
     ref_prim = gko.cons_to_prim(final.reshape(5, -1), 1.4).reshape(5, 1, 256)
     err = rel_l1(o.active(), ref_prim)
     assert np.all(err[[0, 1, 4]] <= 1e-12), err
+
+
+def _solid_body_in(path, prob):
+    """input/solid_body.in's keys at the size and step count of the golden solid-body case."""
+    vals = prob.in_values()
+    for k in [k for k in vals if k.startswith(("selfbd_", "sd0_", "immersed", "shell_"))]:
+        del vals[k]
+    vals.update({"r_solid": prob.shell[3], "x0": prob.shell[0], "y0": prob.shell[1], "z0": prob.shell[2],
+                 "rho": 1.0, "p": 1.0, "vxIn": 4.0})
+    write_in(path, vals)
+
+
+@pytest.mark.parametrize("binary", ["solid_body", "external_solid_body"])
+def test_driver_solid_body_setup(tmp_path, binary):
+    """The solid-body problem through a setup plug-in: this driver's setup_solid_body.cpp, and the
+    reference's own setup file built unchanged on this driver (when its sources were present at
+    build time).  The self-defined inflow face runs as a fixed face, the cylinder as an immersed
+    shell; the result must match the reference's run of the same problem."""
+    path = os.path.join(HOST, "bin", f"gaukuk_{binary}.sim")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not built (the reference setup file was not available)")
+    prob, final, t, steps = load_run("solid96_rk2")
+    _solid_body_in(str(tmp_path / "case.in"), prob)
+    print(f"driver {binary} 96x96", flush=True)
+    r = subprocess.run([path, "-i", "case.in", "--quiet"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    files = sorted(f for f in os.listdir(tmp_path) if f.startswith("cons_"))
+    o = GaukukOutput(str(tmp_path / files[-1]))
+    assert abs(o.t - t) <= 1e-12 * t
+    err = rel_l1(o.active(), final)
+    print(binary, err)
+    assert np.all(err <= 1e-12), err
+
+
+def test_driver_rejects_state_dependent_self_face(tmp_path):
+    prob, _, _, _ = load_run("solid96_rk2")
+    vals = prob.in_values()
+    vals["sdcopy_0"] = 1
+    with pytest.raises(RuntimeError, match="depends on the state"):
+        run_from_file(str(tmp_path), prob, extra_args=("--quiet",), in_values=vals)

@@ -194,3 +194,48 @@ def test_fixed_faces_need_data(hip):
     s = hip.Solver(prob)
     with pytest.raises(hip.GaukukError, match="fixed boundary is not initialized"):
         s.advance(prob.dtoutput)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_solid_body_vs_oracle(hip, precision):
+    """Immersed cylinder + self-defined inflow face (run as a fixed face) at 256^2 (five strips of
+    waves, ~100 shell cells) for 60 RK2 steps."""
+    from oracle import gko
+    prob = P.solid_body(256, stepmax=60)
+    print(f"solid body 256^2 precision={precision}", flush=True)
+    ref = gko.OracleSim(prob, precision=precision).run_to_end()
+    s = hip.Solver(prob, precision=precision).run_to_end()
+    assert s.step == ref.step == 60
+    err = rel_l1(s.download(), ref.cons().astype(np.float64))
+    print("solid body rel L1", err)
+    assert np.all(err <= (TOL64 if precision == 64 else 5e-5)), err
+
+
+@pytest.mark.parametrize("integrator", [1, 3])
+def test_inner_region_vs_oracle(hip, integrator):
+    """Inner fixed region under gravity on a 200x160 grid (four strips), forward Euler (ping-pong
+    buffers) and RK3."""
+    from oracle import gko
+    prob = P.blast(200, 160, p_ratio=20.0, radius=0.25, center=(0.62, 0.5), bc=(1, 1, 2, 2), stepmax=50)
+    x = -1.0 + (np.arange(200) + 0.5) * 0.01
+    y = -0.8 + (np.arange(160) + 0.5) * 0.01
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    z = np.zeros_like(X)
+    cons = P.prim_to_cons(1.0 + 0.3 * np.sin(3 * X) * np.cos(2 * Y), 0.2 + z, z, z,
+                          np.where((X - 0.2) ** 2 + Y ** 2 < 0.1, 2.0, 0.4), 1.4)
+    prob = prob.with_(extent=(-1.0, 1.0, -0.8, 0.8, -0.5, 0.5), integrator=integrator, g=(0.0, -0.2, 0.0),
+                      fix_inner=True, cons=cons)
+    print(f"inner region 200x160 integrator={integrator}", flush=True)
+    ref = gko.OracleSim(prob).run_to_end()
+    s = hip.Solver(prob).run_to_end()
+    assert s.step == ref.step == 50
+    err = rel_l1(s.download(), ref.cons())
+    print("inner rel L1", err)
+    assert np.all(err <= TOL64), err
+
+
+def test_immersed_with_gravity_rejected(hip):
+    prob = P.solid_body(64, stepmax=2).with_(g=(0.0, -0.1, 0.0))
+    s = hip.Solver(prob)
+    with pytest.raises(hip.GaukukError, match="immersed shell together with gravity"):
+        s.advance(prob.dtoutput)
