@@ -5,6 +5,9 @@ It produces, next to this script:
   run_<case>.npz   whole-run fixtures: initial conserved state (active cells), the .in values,
                    the reference's final active state, final time and step count
   vec_<op>.npz     per-function vectors from oracle/_ref/ref_vectors (reference objects)
+  setup_<s>.npz    the reference built with its own setup file s (oracle/_ref/gaukuk_ref_<s>, `make -C
+                   oracle ref-setups`) run on its own input file at a reduced size: the .in values,
+                   the last output file's data, its type, final time and step count
 The reference binary reads its initial state through oracle/ref_setup_from_file.cpp, so the
 ICs are exactly the arrays stored in each fixture.
 """
@@ -23,7 +26,7 @@ ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
 sys.path.insert(0, ROOT)
 
 from gaukuk_amd import problems as P  # noqa: E402
-from gaukuk_amd.config import write_in  # noqa: E402
+from gaukuk_amd.config import Config, write_in  # noqa: E402
 from gaukuk_amd.gkio import GaukukOutput  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -203,11 +206,52 @@ def vectors(rng):
                         cmax=cm)
 
 
+# reference setups on their own input files, 2D at reduced size (NZ = 1; the 3D inputs blast.in and
+# shock_tube.in in their 2D form)
+SETUP_SIZES = {
+    "blast": dict(NX=64, NY=64, NZ=1, stepmax=20),
+    "disk": dict(NX=96, NY=96, stepmax=20),
+    "disk_planet": dict(NX=96, NY=96, stepmax=20),
+    "kh": dict(NX=64, NY=64, stepmax=20),
+    "rt": dict(NX=64, NY=128, stepmax=20),
+    "shock_tube": dict(NX=128, NY=4, NZ=1, stepmax=20),
+    "solid_body": dict(NX=96, NY=96, stepmax=20),
+    "wave_test": dict(NX=64, NY=64, stepmax=20),
+}
+
+
+def reference_setups():
+    """Each reference setup file, built into the reference itself, on its shipped input file."""
+    for name, over in SETUP_SIZES.items():
+        cfg = Config.load(os.path.join("/root/reference/input", f"{name}.in"))
+        vals = dict(cfg.data)
+        vals.update(over)
+        vals["tmax"] = 1e30
+        vals["dtoutput"] = 1e30
+        with tempfile.TemporaryDirectory() as d:
+            write_in(os.path.join(d, "case.in"), vals)
+            r = subprocess.run([os.path.join(REF, f"gaukuk_ref_{name}"), "-i", "case.in"], cwd=d,
+                               env=dict(os.environ, OMP_NUM_THREADS="4"), capture_output=True, text=True,
+                               check=True)
+            steps = int(re.search(r"total steps =\s*(\d+)", r.stdout).group(1))
+            kind = "prim" if int(vals.get("dataType", 0)) == 1 else "cons"
+            outs = sorted(f for f in os.listdir(d) if f.startswith(kind + "_"))
+            last = GaukukOutput(os.path.join(d, outs[-1]))
+            np.savez_compressed(os.path.join(HERE, f"setup_{name}.npz"), final=last.active().copy(),
+                                t=np.array(float(last.t)), steps=np.array(steps), kind=kind,
+                                meta=json.dumps(vals))
+        print(f"setup {name:14s} {vals['NX']}x{vals['NY']} steps={steps} t={float(last.t)!r} ({kind})")
+
+
 def main():
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default="", help="comma-separated run cases to (re)make; default: everything")
+    ap.add_argument("--setups", action="store_true", help="(re)make only the reference-setup fixtures")
     a = ap.parse_args()
+    if a.setups:
+        reference_setups()
+        return
     only = [v for v in a.only.split(",") if v]
     if not os.path.exists(os.path.join(REF, "gaukuk_ref")):
         sys.exit("oracle/_ref/gaukuk_ref missing: run `make -C oracle ref` where the reference exists")
