@@ -72,8 +72,10 @@ class GaukukOutput:
             self.data = np.fromfile(f, dtype, self.dataLen).reshape(self.nvar, self.lenz, self.leny, self.lenx)
 
     def active(self) -> np.ndarray:
-        """Active cells of the first active z plane, shape (NVar, ny, nx)."""
+        """Active cells: (NVar, ny, nx) in 2D, (NVar, nz, ny, nx) when nz > 1."""
         g = self.nGhost
+        if self.nz > 1:
+            return self.data[:, g:g + self.nz, g:g + self.ny, g:g + self.nx]
         return self.data[:, g, g:g + self.ny, g:g + self.nx]
 
 

@@ -1,5 +1,5 @@
 // setup_from_file.cpp -- active-region conserved state from the binary file named by
-// $GK_IC_FILE: int32 nvar (5), int32 nx, int32 ny, float64 cons[5][ny][nx]; config keys
+// $GK_IC_FILE: int32 nvar (5), int32 nx, int32 ny, float64 cons[5][nz][ny][nx]; config keys
 // gx, gy, gz enroll constant gravity; npoint and pg<q>_* enroll point masses; fixedbd = 1 fills
 // every ghost cell with the conserved state fix_0..fix_4, fixedbd = 2 with the plane read from
 // $GK_FIXED_FILE, and enrolls it as fixed-face data (fixinner = 1: with the inner region);
@@ -80,7 +80,7 @@ void Sim::Setup() {
         std::fclose(f);
         throw std::runtime_error("IC file header does not match NX/NY");
     }
-    const size_t n = (size_t)NVar * grid.nx * grid.ny;
+    const size_t n = (size_t)NVar * grid.nx * grid.ny * grid.nz;
     std::vector<double> buf(n);
     const size_t got = std::fread(buf.data(), sizeof(double), n, f);
     std::fclose(f);
@@ -96,9 +96,11 @@ void Sim::Setup() {
                     for (int i = 0; i < grid.lenx; i++) cons(v, k, j, i) = fv;
         }
     for (int v = 0; v < NVar; v++)
-        for (int j = 0; j < grid.ny; j++)
-            for (int i = 0; i < grid.nx; i++)
-                cons(v, grid.kb, grid.jb + j, grid.ib + i) = (Real)buf[((size_t)v * grid.ny + j) * grid.nx + i];
+        for (int k = 0; k < grid.nz; k++)
+            for (int j = 0; j < grid.ny; j++)
+                for (int i = 0; i < grid.nx; i++)
+                    cons(v, grid.kb + k, grid.jb + j, grid.ib + i) =
+                        (Real)buf[(((size_t)v * grid.nz + k) * grid.ny + j) * grid.nx + i];
     const bool fixinner = (int)c.get("fixinner", 0.0) == 1;
     if (fixedbd == 1 || fixedbd == 2 || fixinner) boundary.EnrollFixedBoundaryData(cons, grid, fixinner);
     if ((int)c.get("selfbd_0", 0.0) == 1) boundary.EnrollSelfDefineBDXL();

@@ -51,6 +51,8 @@ class Problem:
     tmax: float = 1e30
     dtoutput: float = 1e30
     bc: tuple = (1, 1, 1, 1)  # xleft, xright, yleft, yright
+    nz: int = 1               # active z planes; > 1 is the 3D path (cons is then (5, nz, ny, nx))
+    bcz: tuple = (1, 1)       # zleft, zright (read only in 3D)
     g: tuple = (0.0, 0.0, 0.0)
     # point masses, each (gm, x, y, z, vx, vy, vz, rs) as SourceTerm::EnrollPointGravity takes them
     # (src/source_term/source.cpp:33-67); two masses = binary gravity
@@ -151,9 +153,9 @@ class Problem:
             "gamma": self.gamma, "tmax": self.tmax, "stepmax": self.stepmax, "CFL": self.cfl,
             "integrator": self.integrator, "RcOrder": self.rc_order, "dtoutput": self.dtoutput,
             "dataType": 0, "xmin": x0, "xmax": x1, "ymin": y0, "ymax": y1, "zmin": z0, "zmax": z1,
-            "NX": self.nx, "NY": self.ny, "NZ": 1, "NGhost": self.ng,
+            "NX": self.nx, "NY": self.ny, "NZ": self.nz, "NGhost": self.ng,
             "xleft": self.bc[0], "xright": self.bc[1], "yleft": self.bc[2], "yright": self.bc[3],
-            "zleft": 1, "zright": 1, "gx": self.g[0], "gy": self.g[1], "gz": self.g[2],
+            "zleft": self.bcz[0], "zright": self.bcz[1], "gx": self.g[0], "gy": self.g[1], "gz": self.g[2],
             **self._point_keys(),
         }
 
@@ -269,6 +271,47 @@ def solid_body(nx: int = 1024, ny: int | None = None, rho: float = 1.0, p: float
                 bc=(3, 3, 3, 3), self_faces=((0, inflow),), shell=(x0, y0, z0, r_solid))
     base.update(kw)
     return Problem(cons=cons, **base)
+
+
+def blast3d(nx: int, ny: int, nz: int, p_ratio: float = 30.0, radius: float = 0.15, center=None,
+            bc=1, bcz=(1, 1), rho0: float = 1.0, p_amb: float = 0.1, **kw) -> Problem:
+    """3D spherical overpressure on [0, nx/L] x [0, ny/L] x [0, nz/L], L = max(nx, ny, nz), with a
+    density gradient along x so that every axis carries structure."""
+    L = float(max(nx, ny, nz))
+    ext = (0.0, nx / L, 0.0, ny / L, 0.0, nz / L)
+    cx, cy, cz = center if center is not None else (0.5 * nx / L, 0.5 * ny / L, 0.5 * nz / L)
+    x = (np.arange(nx) + 0.5) / L
+    y = (np.arange(ny) + 0.5) / L
+    z = (np.arange(nz) + 0.5) / L
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    r = np.sqrt((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2)
+    rho = rho0 * (1.0 + 0.2 * X)
+    p = np.where(r < radius, p_amb * p_ratio, p_amb)
+    zero = np.zeros_like(rho)
+    bcs = (bc,) * 4 if isinstance(bc, int) else tuple(bc)
+    base = dict(name="blast3d", nx=nx, ny=ny, nz=nz, extent=ext, integrator=2, rc_order=2, cfl=0.4, bc=bcs,
+                bcz=tuple(bcz))
+    base.update(kw)
+    return Problem(cons=prim_to_cons(rho, zero, zero, zero, p, base.get("gamma", 1.4)), **base)
+
+
+def shear3d(n: int, nz: int, seed: int = 0, amp: float = 0.02, **kw) -> Problem:
+    """3D Kelvin-Helmholtz: the configuration's shear layer (rho 2 / 1, vx -+0.5, p 2.5) on
+    [0,1]^2 x [0, nz/n], with a seeded perturbation in vy and vz."""
+    ext = (0.0, 1.0, 0.0, 1.0, 0.0, nz / n)
+    x = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid((np.arange(nz) + 0.5) / n, x, x, indexing="ij")
+    inner = np.abs(Y - 0.5) < 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    vx = np.where(inner, 0.5, -0.5)
+    idx = np.arange(n * n * nz).reshape(nz, n, n)
+    vy = amp * np.sin(4 * np.pi * X) + amp * uniform_pm1(seed, idx, 0)
+    vz = amp * uniform_pm1(seed, idx, 1)
+    p = np.full_like(rho, 2.5)
+    base = dict(name="shear3d", nx=n, ny=n, nz=nz, extent=ext, integrator=3, rc_order=2, cfl=0.4, bc=(1, 1, 1, 1),
+                bcz=(1, 1))
+    base.update(kw)
+    return Problem(cons=prim_to_cons(rho, vx, vy, vz, p, base.get("gamma", 1.4)), **base)
 
 
 def riemann_1d(nx: int, left, right, x0: float = 0.5, **kw) -> Problem:

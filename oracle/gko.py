@@ -19,8 +19,8 @@ class Params(C.Structure):
                 ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
                 ("gamma", C.c_double), ("cfl", C.c_double),
                 ("integrator", C.c_int), ("rc_order", C.c_int), ("stepmax", C.c_int),
-                ("bc", C.c_int * 4), ("g", C.c_double * 3),
-                ("npoint", C.c_int), ("pg", (C.c_double * 8) * 2)]
+                ("bc", C.c_int * 6), ("g", C.c_double * 3),
+                ("npoint", C.c_int), ("pg", (C.c_double * 8) * 2), ("nz", C.c_int)]
 
 
 _libs = {}
@@ -70,6 +70,8 @@ def make_params(prob) -> Params:
     p.integrator, p.rc_order, p.stepmax = prob.integrator, prob.rc_order, prob.stepmax
     for k in range(4):
         p.bc[k] = prob.device_bc()[k]
+    p.bc[4], p.bc[5] = prob.bcz
+    p.nz = prob.nz
     for k in range(3):
         p.g[k] = prob.g[k]
     p.npoint = len(prob.point)
@@ -92,12 +94,14 @@ class OracleSim:
             raise ValueError("oracle rejected the parameters")
         ng = prob.ng
         self.shape = (5, prob.ny + 2 * ng, prob.nx + 2 * ng)
+        if prob.nz > 1:
+            self.shape = (5, prob.nz + 2 * ng) + self.shape[1:]
         full = np.zeros(self.shape, self.dtype)
         enrolled = prob.enrolled_state()
         if enrolled is not None:
             # a setup that fills the ghost cells, then enrolls the fixed data
             full[:] = enrolled
-        full[:, ng:ng + prob.ny, ng:ng + prob.nx] = prob.cons
+        full[self._active()] = prob.cons
         self.L.gko_set_cons(self.h, full.ctypes.data)
         if enrolled is not None:
             self.L.gko_enroll_fixed(self.h, int(bool(prob.fix_inner)))
@@ -121,14 +125,18 @@ class OracleSim:
         self.L.gko_enroll_fixed(self.h, 0)
         self.L.gko_set_cons(self.h, cur.ctypes.data)
 
+    def _active(self):
+        g, p = self.prob.ng, self.prob
+        act = (slice(None), slice(g, g + p.ny), slice(g, g + p.nx))
+        return act[:1] + (slice(g, g + p.nz),) + act[1:] if p.nz > 1 else act
+
     def full(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         self.L.gko_get_cons(self.h, out.ctypes.data)
         return out
 
     def cons(self) -> np.ndarray:
-        ng = self.prob.ng
-        return self.full()[:, ng:ng + self.prob.ny, ng:ng + self.prob.nx].copy()
+        return self.full()[self._active()].copy()
 
     def update_bd(self):
         """Boundary::UpdateBD on the current state, as main does once after Setup (src/main.cpp:30):

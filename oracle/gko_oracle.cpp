@@ -25,28 +25,33 @@ template <class R> inline R smax(R a, R b) { return (a < b) ? b : a; }
 template <class Real>
 struct Oracle {
     gko_params P;
-    int nx, ny, ng, lenx, leny, ib, ie, jb, je;
+    int nx, ny, nz, ng, lenx, leny, lenz, rows, ib, ie, jb, je, kb, ke;
     Real gamma, gm1Rec, cfl;
-    Real dx, dy, dz, dxRec, dyRec, drmin;
+    Real dx, dy, dz, dxRec, dyRec, dzRec, drmin;
     Real gx, gy, gz;
     bool src_enrolled, has_const, has_point, has_binary;
     // point masses (GravitySource, src/source_term/source.hpp:11-20) and the binary's orbit
     struct Mass { Real gm, x, y, z, vx, vy, vz, rs; } obj0{}, obj1{};
     Real ab_ = 0, omega_ = 0, theta1_ = 0;
     std::vector<Real> xc, yc;   // cell centres (Domain, src/sim.cpp:37-39), ghost-inclusive index
-    Real zc0 = 0;               // z centre of the active plane
+    Real zc0 = 0;               // z centre of the active plane (2D)
+    std::vector<Real> zc;       // 3D: z centres, ghost-inclusive
     // state
-    std::vector<Real> cons, prim, flx1, flx2, consTemp;
+    std::vector<Real> cons, prim, flx1, flx2, flx3, consTemp;
     Real t, dt, dtUntilOutput, cmax;
     int step;
     bool isContinue;
     std::vector<double> log_t, log_dt;
 
     explicit Oracle(const gko_params& p) : P(p) {
-        nx = p.nx; ny = p.ny; ng = p.ng;
+        nx = p.nx; ny = p.ny; nz = p.nz < 1 ? 1 : p.nz; ng = p.ng;
         if (nx <= 0 || ny <= 0 || ng < 0) throw std::runtime_error("Invalid grid parameters");
         lenx = nx + 2 * ng; leny = ny + 2 * ng;
+        // 2D keeps only the active z plane; 3D keeps every plane (Grid: src/grid.hpp:23-35)
+        lenz = nz > 1 ? nz + 2 * ng : 1;
+        rows = lenz * leny;
         ib = ng; ie = ib + nx; jb = ng; je = jb + ny;
+        kb = nz > 1 ? ng : 0; ke = kb + nz;
         // EquationOfState ctor: src/eos/adiabatic.cpp:12-16
         gamma = (Real)p.gamma;
         gm1Rec = Real(1.0) / (gamma - Real(1.0));
@@ -54,10 +59,11 @@ struct Oracle {
         // Domain ctor: src/sim.cpp:16-36 (nz == 1)
         dx = ((Real)p.xmax - (Real)p.xmin) / (Real)nx;
         dy = ((Real)p.ymax - (Real)p.ymin) / (Real)ny;
-        dz = ((Real)p.zmax - (Real)p.zmin) / (Real)1;
+        dz = ((Real)p.zmax - (Real)p.zmin) / (Real)nz;
         dxRec = Real(1.) / dx;
         dyRec = Real(1.) / dy;
         if (ny <= 1) dyRec = 0;
+        dzRec = nz > 1 ? Real(1.) / dz : Real(0);
         drmin = smin(smin(dx, dy), dz);
         // SourceTerm::EnrollConstGravityVector: src/source_term/source.cpp:23-31
         gx = (Real)p.g[0]; gy = (Real)p.g[1]; gz = (Real)p.g[2];
@@ -70,44 +76,59 @@ struct Oracle {
         for (int i = 0; i < lenx; i++) xc[i] = (Real)p.xmin - (Real)ng * dx + (Real)0.5 * dx + (Real)i * dx;
         for (int j = 0; j < leny; j++) yc[j] = (Real)p.ymin - (Real)ng * dy + (Real)0.5 * dy + (Real)j * dy;
         zc0 = (Real)p.zmin - (Real)ng * dz + (Real)0.5 * dz + (Real)ng * dz;
+        zc.resize(nz + 2 * ng);
+        for (int k = 0; k < nz + 2 * ng; k++) zc[k] = (Real)p.zmin - (Real)ng * dz + (Real)0.5 * dz + (Real)k * dz;
         // SourceTerm::EnrollPointGravity, called npoint times: src/source_term/source.cpp:33-67
         for (int q = 0; q < p.npoint && q < 2; q++) {
             const double* g = p.pg[q];
             EnrollPointGravity((Real)g[0], (Real)g[1], (Real)g[2], (Real)g[3], (Real)g[4], (Real)g[5], (Real)g[6],
                                (Real)g[7]);
         }
-        size_t n = (size_t)NV * leny * lenx;
+        size_t n = (size_t)NV * rows * lenx;
         cons.assign(n, 0); prim.assign(n, 0); consTemp.assign(n, 0);
-        flx1.assign((size_t)NV * leny * (lenx + 1), 0);
-        flx2.assign((size_t)NV * (leny + 1) * lenx, 0);
+        flx1.assign((size_t)NV * rows * (lenx + 1), 0);
+        flx2.assign((size_t)NV * lenz * (leny + 1) * lenx, 0);
+        if (nz > 1) flx3.assign((size_t)NV * (lenz + 1) * leny * lenx, 0);
         // Sim ctor: src/sim.cpp:45-58
         t = 0; dt = (Real)1e10; dtUntilOutput = (Real)1e10; cmax = (Real)1e-16;
         step = 0; isContinue = true;
     }
 
-    inline size_t at(int n, int j, int i) const { return (size_t)i + (size_t)lenx * ((size_t)j + (size_t)leny * n); }
-    inline size_t at1(int n, int j, int i) const { return (size_t)i + (size_t)(lenx + 1) * ((size_t)j + (size_t)leny * n); }
-    inline size_t at2(int n, int j, int i) const { return (size_t)i + (size_t)lenx * ((size_t)j + (size_t)(leny + 1) * n); }
+    // Arrays are [n][k][j][i] (x fastest); `j` below is a row of plane k, k = 0 in 2D.
+    inline size_t at(int n, int j, int i, int k = 0) const {
+        return (size_t)i + (size_t)lenx * ((size_t)j + (size_t)leny * ((size_t)k + (size_t)lenz * n));
+    }
+    inline size_t at1(int n, int j, int i, int k = 0) const {
+        return (size_t)i + (size_t)(lenx + 1) * ((size_t)j + (size_t)leny * ((size_t)k + (size_t)lenz * n));
+    }
+    inline size_t at2(int n, int j, int i, int k = 0) const {
+        return (size_t)i + (size_t)lenx * ((size_t)j + (size_t)(leny + 1) * ((size_t)k + (size_t)lenz * n));
+    }
+    inline size_t at3(int n, int k, int j, int i) const {
+        return (size_t)i + (size_t)lenx * ((size_t)j + (size_t)leny * ((size_t)k + (size_t)(lenz + 1) * n));
+    }
 
     // ---- EOS -------------------------------------------------------------------------
     // ConsToPrim over all cells incl. ghosts: src/eos/adiabatic.cpp:18-61
     void ConsToPrim(const std::vector<Real>& U, std::vector<Real>& W) {
         const Real gm1 = gamma - Real(1.0);
         const Real dmin = (Real)1e-16, pmin = (Real)1e-16;
+        for (int k = 0; k < lenz; k++) {
 #pragma omp parallel for schedule(static)
-        for (int j = 0; j < leny; j++) {
-            for (int i = 0; i < lenx; i++) {
-                Real d = U[at(DEN, j, i)], m1 = U[at(MX, j, i)], m2 = U[at(MY, j, i)],
-                     m3 = U[at(MZ, j, i)], e = U[at(ENG, j, i)];
-                d = smax(d, dmin);
-                W[at(DEN, j, i)] = d;
-                Real inv = Real(1.0) / d;
-                W[at(VX, j, i)] = m1 * inv;
-                W[at(VY, j, i)] = m2 * inv;
-                W[at(VZ, j, i)] = m3 * inv;
-                Real ek = Real(0.5) * inv * (m1 * m1 + m2 * m2 + m3 * m3);
-                Real p = gm1 * (e - ek);
-                W[at(PRE, j, i)] = (p > pmin) ? p : pmin;
+            for (int j = 0; j < leny; j++) {
+                for (int i = 0; i < lenx; i++) {
+                    Real d = U[at(DEN, j, i, k)], m1 = U[at(MX, j, i, k)], m2 = U[at(MY, j, i, k)],
+                         m3 = U[at(MZ, j, i, k)], e = U[at(ENG, j, i, k)];
+                    d = smax(d, dmin);
+                    W[at(DEN, j, i, k)] = d;
+                    Real inv = Real(1.0) / d;
+                    W[at(VX, j, i, k)] = m1 * inv;
+                    W[at(VY, j, i, k)] = m2 * inv;
+                    W[at(VZ, j, i, k)] = m3 * inv;
+                    Real ek = Real(0.5) * inv * (m1 * m1 + m2 * m2 + m3 * m3);
+                    Real p = gm1 * (e - ek);
+                    W[at(PRE, j, i, k)] = (p > pmin) ? p : pmin;
+                }
             }
         }
     }
@@ -117,11 +138,12 @@ struct Oracle {
         const Real gm1 = gamma - Real(1.0);
         const Real dmin = (Real)1e-16, pmin = (Real)1e-16;
         Real best = cm;
+        for (int k = kb; k < ke; k++) {
 #pragma omp parallel for schedule(static) reduction(max : best)
         for (int j = jb; j < je; j++) {
             for (int i = ib; i < ie; i++) {
-                Real d = U[at(DEN, j, i)], m1 = U[at(MX, j, i)], m2 = U[at(MY, j, i)],
-                     m3 = U[at(MZ, j, i)], e = U[at(ENG, j, i)];
+                Real d = U[at(DEN, j, i, k)], m1 = U[at(MX, j, i, k)], m2 = U[at(MY, j, i, k)],
+                     m3 = U[at(MZ, j, i, k)], e = U[at(ENG, j, i, k)];
                 d = smax(d, dmin);
                 Real inv = Real(1.0) / d;
                 Real v1 = m1 * inv, v2 = m2 * inv, v3 = m3 * inv;
@@ -135,6 +157,7 @@ struct Oracle {
                 best = smax(best, loc);
             }
         }
+        }
         cm = best;
     }
 
@@ -143,69 +166,76 @@ struct Oracle {
     // 2: reflective.cpp:19-166; 3: outflow.cpp:20-213.  Dispatch order Bdxl, Bdxr, Bdyl,
     // Bdyr: src/boundary/boundary.cpp:127-142 (z faces only when nz > 1: not here).
     // Unknown codes fall back to type 0 (boundary.cpp:27-29 and siblings).
+    // One ghost cell g of a face filled from source cell s (types 1-3; type 0 copies the edge cell):
+    // comp is the normal momentum component, lowSide selects the outflow clamp's sign.
+    void FillCell(std::vector<Real>& U, size_t g[NV], size_t s[NV], int type, int comp, bool lowSide) {
+        if (type == 2 || type == 3) {
+            Real d = U[s[DEN]], m1 = U[s[MX]], m2 = U[s[MY]], m3 = U[s[MZ]], e = U[s[ENG]];
+            Real m[3] = {m1, m2, m3};
+            Real& mn = m[comp - MX];
+            if (type == 2) {
+                mn = -mn;
+                U[g[DEN]] = d; U[g[MX]] = m[0]; U[g[MY]] = m[1]; U[g[MZ]] = m[2]; U[g[ENG]] = e;
+            } else {
+                const Real mo = mn;
+                Real mc = lowSide ? smin(mo, Real(0.0)) : smax(mo, Real(0.0));
+                d = smax(d, (Real)1e-16);
+                Real inv = Real(1.0) / d;
+                Real keOld = Real(0.5) * inv * mo * mo;
+                Real keNew = Real(0.5) * inv * mc * mc;
+                Real eint = e - keOld;
+                eint = smax(eint, (Real)1e-16);
+                mn = mc;
+                U[g[DEN]] = d; U[g[MX]] = m[0]; U[g[MY]] = m[1]; U[g[MZ]] = m[2]; U[g[ENG]] = eint + keNew;
+            }
+        } else {
+            for (int n = 0; n < NV; n++) U[g[n]] = U[s[n]];
+        }
+    }
     void FillX(std::vector<Real>& U, bool left, int type) {
         const int gl = left ? 0 : ie, gr = left ? ib : lenx;
-        for (int j = jb; j < je; j++) {
-            for (int i = gl; i < gr; i++) {
-                if (type == 1) {
-                    int src = left ? i + nx : i - nx;
-                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, j, src)];
-                } else if (type == 2 || type == 3) {
-                    int src = left ? (ib + ib - i - 1) : (ie + ie - i - 1);
-                    Real d = U[at(DEN, j, src)], m1 = U[at(MX, j, src)], m2 = U[at(MY, j, src)],
-                         m3 = U[at(MZ, j, src)], e = U[at(ENG, j, src)];
-                    if (type == 2) {
-                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = -m1; U[at(MY, j, i)] = m2;
-                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = e;
-                    } else {
-                        Real mc = left ? smin(m1, Real(0.0)) : smax(m1, Real(0.0));
-                        d = smax(d, (Real)1e-16);
-                        Real inv = Real(1.0) / d;
-                        Real keOld = Real(0.5) * inv * m1 * m1;
-                        Real keNew = Real(0.5) * inv * mc * mc;
-                        Real eint = e - keOld;
-                        eint = smax(eint, (Real)1e-16);
-                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = mc; U[at(MY, j, i)] = m2;
-                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = eint + keNew;
-                    }
-                } else {
+        for (int k = kb; k < ke; k++)
+            for (int j = jb; j < je; j++)
+                for (int i = gl; i < gr; i++) {
                     int src = left ? ib : ie - 1;
-                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, j, src)];
+                    if (type == 1) src = left ? i + nx : i - nx;
+                    else if (type == 2 || type == 3) src = left ? (ib + ib - i - 1) : (ie + ie - i - 1);
+                    size_t g[NV], sv[NV];
+                    for (int n = 0; n < NV; n++) { g[n] = at(n, j, i, k); sv[n] = at(n, j, src, k); }
+                    FillCell(U, g, sv, type, MX, left);
                 }
-            }
-        }
     }
     void FillY(std::vector<Real>& U, bool left, int type) {
         const int gl = left ? 0 : je, gr = left ? jb : leny;
-        for (int j = gl; j < gr; j++) {
-            for (int i = ib; i < ie; i++) {
-                if (type == 1) {
-                    int src = left ? j + ny : j - ny;
-                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, src, i)];
-                } else if (type == 2 || type == 3) {
-                    int src = left ? (jb + jb - j - 1) : (je + je - j - 1);
-                    Real d = U[at(DEN, src, i)], m1 = U[at(MX, src, i)], m2 = U[at(MY, src, i)],
-                         m3 = U[at(MZ, src, i)], e = U[at(ENG, src, i)];
-                    if (type == 2) {
-                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = m1; U[at(MY, j, i)] = -m2;
-                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = e;
-                    } else {
-                        Real mc = left ? smin(m2, Real(0.0)) : smax(m2, Real(0.0));
-                        d = smax(d, (Real)1e-16);
-                        Real inv = Real(1.0) / d;
-                        Real keOld = Real(0.5) * inv * m2 * m2;
-                        Real keNew = Real(0.5) * inv * mc * mc;
-                        Real eint = e - keOld;
-                        eint = smax(eint, (Real)1e-16);
-                        U[at(DEN, j, i)] = d; U[at(MX, j, i)] = m1; U[at(MY, j, i)] = mc;
-                        U[at(MZ, j, i)] = m3; U[at(ENG, j, i)] = eint + keNew;
-                    }
-                } else {
+        for (int k = kb; k < ke; k++)
+            for (int j = gl; j < gr; j++)
+                for (int i = ib; i < ie; i++) {
                     int src = left ? jb : je - 1;
-                    for (int n = 0; n < NV; n++) U[at(n, j, i)] = U[at(n, src, i)];
+                    if (type == 1) src = left ? j + ny : j - ny;
+                    else if (type == 2 || type == 3) src = left ? (jb + jb - j - 1) : (je + je - j - 1);
+                    size_t g[NV], sv[NV];
+                    for (int n = 0; n < NV; n++) { g[n] = at(n, j, i, k); sv[n] = at(n, src, i, k); }
+                    FillCell(U, g, sv, type, MY, left);
                 }
-            }
+    }
+    // z faces (3D): types 1-3 over the active (j, i) range (periodic.cpp:135-190,
+    // reflective.cpp:170-230, outflow.cpp:216-300).  Type 0 and unknown codes register the *y* copy
+    // functions (boundary.cpp:90-122: Bdzl = &OutflowCopyYL, Bdzr = &OutflowCopyYR), so the z ghost
+    // planes are left as they are and the y ghosts are copied once more.
+    void FillZ(std::vector<Real>& U, bool left, int type) {
+        if (type != 1 && type != 2 && type != 3) {
+            FillY(U, left, 0);
+            return;
         }
+        const int gl = left ? 0 : ke, gr = left ? kb : lenz;
+        for (int k = gl; k < gr; k++)
+            for (int j = jb; j < je; j++)
+                for (int i = ib; i < ie; i++) {
+                    int src = type == 1 ? (left ? k + nz : k - nz) : (left ? (kb + kb - k - 1) : (ke + ke - k - 1));
+                    size_t g[NV], sv[NV];
+                    for (int n = 0; n < NV; n++) { g[n] = at(n, j, i, k); sv[n] = at(n, j, i, src); }
+                    FillCell(U, g, sv, type, MZ, left);
+                }
     }
     // Type 4, fixed: src/boundary/fixed.cpp:16-127 -- the ghost layers saved by
     // EnrollFixedBoundaryData (fixed.cpp:242-269) are copied back over the active tangential
@@ -393,10 +423,17 @@ struct Oracle {
     // Boundary::UpdateBD: src/boundary/boundary.cpp:127-142 -- faces, then the immersed shell,
     // then the inner fixed region
     void UpdateBD(std::vector<Real>& U) {
+        if (nz > 1 && (P.bc[0] == 4 || P.bc[1] == 4 || P.bc[2] == 4 || P.bc[3] == 4 || P.bc[4] == 4 ||
+                       P.bc[5] == 4 || immersed || !fixedInner.empty()))
+            throw std::invalid_argument("3D runs take face types 0-3 only here");
         if (P.bc[0] == 4) FillFixedX(U, true); else FillX(U, true, P.bc[0]);
         if (P.bc[1] == 4) FillFixedX(U, false); else FillX(U, false, P.bc[1]);
         if (P.bc[2] == 4) FillFixedY(U, true); else FillY(U, true, P.bc[2]);
         if (P.bc[3] == 4) FillFixedY(U, false); else FillY(U, false, P.bc[3]);
+        if (nz > 1) {
+            FillZ(U, true, P.bc[4]);
+            FillZ(U, false, P.bc[5]);
+        }
         if (immersed) ImmersedReflective(U);
         if (!fixedInner.empty()) FixedBDInner(U);
     }
@@ -452,93 +489,138 @@ struct Oracle {
     }
 
     // Left state at face f from cell f-1, right state from cell f (PLM: recon_piecewise_linear.cpp:8-55;
-    // first order: recon_first_order.cpp:8-29).  dir 1 = x (cells along i), 2 = y (cells along j).
-    inline Real CellVal(int n, int j, int i) const { return prim[at(n, j, i)]; }
+    // first order: recon_first_order.cpp:8-29).
+    inline Real CellVal(int n, int j, int i, int k = 0) const { return prim[at(n, j, i, k)]; }
 
-    // x sweep over active rows, faces ib..ie (the faces UpdateCons reads): cal_flux.cpp:90-101
+    // x sweep over active rows (and planes), faces ib..ie (the faces UpdateCons reads): cal_flux.cpp:90-101
     void SweepX() {
         const int rotate[NV] = {DEN, VX, VY, VZ, PRE};
+        for (int k = kb; k < ke; k++) {
 #pragma omp parallel for schedule(static)
-        for (int j = jb; j < je; j++) {
-            Real wl[NV], wr[NV], f[NV];
-            for (int fi = ib; fi <= ie; fi++) {
-                for (int q = 0; q < NV; q++) {
-                    int n = rotate[q];
-                    if (P.rc_order == 1) {
-                        wl[q] = CellVal(n, j, fi - 1);
-                        wr[q] = CellVal(n, j, fi);
-                    } else {
-                        Real a0 = CellVal(n, j, fi - 2), a1 = CellVal(n, j, fi - 1),
-                             a2 = CellVal(n, j, fi), a3 = CellVal(n, j, fi + 1);
-                        Real sl = VanLeer(a1 - a0, a2 - a1);
-                        Real sr = VanLeer(a2 - a1, a3 - a2);
-                        wl[q] = a1 + Real(0.5) * sl;
-                        wr[q] = a2 - Real(0.5) * sr;
+            for (int j = jb; j < je; j++) {
+                Real wl[NV], wr[NV], f[NV];
+                for (int fi = ib; fi <= ie; fi++) {
+                    for (int q = 0; q < NV; q++) {
+                        int n = rotate[q];
+                        if (P.rc_order == 1) {
+                            wl[q] = CellVal(n, j, fi - 1, k);
+                            wr[q] = CellVal(n, j, fi, k);
+                        } else {
+                            Real a0 = CellVal(n, j, fi - 2, k), a1 = CellVal(n, j, fi - 1, k),
+                                 a2 = CellVal(n, j, fi, k), a3 = CellVal(n, j, fi + 1, k);
+                            Real sl = VanLeer(a1 - a0, a2 - a1);
+                            Real sr = VanLeer(a2 - a1, a3 - a2);
+                            wl[q] = a1 + Real(0.5) * sl;
+                            wr[q] = a2 - Real(0.5) * sr;
+                        }
                     }
+                    Hllc(wl, wr, f);
+                    for (int q = 0; q < NV; q++) flx1[at1(rotate[q], j, fi, k)] = f[q];
                 }
-                Hllc(wl, wr, f);
-                for (int q = 0; q < NV; q++) flx1[at1(rotate[q], j, fi)] = f[q];
             }
         }
     }
 
-    // y sweep, faces jb..je over active columns: cal_flux.cpp:103-121.  HLLC rotation for
-    // direction VLY (hllc.cpp:25-27): normal vy, tangents (vz, vx).
+    // y sweep, faces jb..je over active columns (and planes): cal_flux.cpp:103-121 (2D),
+    // 123-138 (3D).  HLLC rotation for direction VLY (hllc.cpp:25-27): normal vy, tangents (vz, vx).
     void SweepY() {
         const int rotate[NV] = {DEN, VY, VZ, VX, PRE};
+        for (int k = kb; k < ke; k++) {
 #pragma omp parallel for schedule(static)
-        for (int fj = jb; fj <= je; fj++) {
-            Real wl[NV], wr[NV], f[NV];
-            for (int i = ib; i < ie; i++) {
-                for (int q = 0; q < NV; q++) {
-                    int n = rotate[q];
-                    if (P.rc_order == 1) {
-                        wl[q] = CellVal(n, fj - 1, i);
-                        wr[q] = CellVal(n, fj, i);
-                    } else {
-                        Real a0 = CellVal(n, fj - 2, i), a1 = CellVal(n, fj - 1, i),
-                             a2 = CellVal(n, fj, i), a3 = CellVal(n, fj + 1, i);
-                        Real sl = VanLeer(a1 - a0, a2 - a1);
-                        Real sr = VanLeer(a2 - a1, a3 - a2);
-                        wl[q] = a1 + Real(0.5) * sl;
-                        wr[q] = a2 - Real(0.5) * sr;
+            for (int fj = jb; fj <= je; fj++) {
+                Real wl[NV], wr[NV], f[NV];
+                for (int i = ib; i < ie; i++) {
+                    for (int q = 0; q < NV; q++) {
+                        int n = rotate[q];
+                        if (P.rc_order == 1) {
+                            wl[q] = CellVal(n, fj - 1, i, k);
+                            wr[q] = CellVal(n, fj, i, k);
+                        } else {
+                            Real a0 = CellVal(n, fj - 2, i, k), a1 = CellVal(n, fj - 1, i, k),
+                                 a2 = CellVal(n, fj, i, k), a3 = CellVal(n, fj + 1, i, k);
+                            Real sl = VanLeer(a1 - a0, a2 - a1);
+                            Real sr = VanLeer(a2 - a1, a3 - a2);
+                            wl[q] = a1 + Real(0.5) * sl;
+                            wr[q] = a2 - Real(0.5) * sr;
+                        }
                     }
+                    Hllc(wl, wr, f);
+                    for (int q = 0; q < NV; q++) flx2[at2(rotate[q], fj, i, k)] = f[q];
                 }
-                Hllc(wl, wr, f);
-                for (int q = 0; q < NV; q++) flx2[at2(rotate[q], fj, i)] = f[q];
             }
         }
     }
 
-    void CalFlux() { SweepX(); SweepY(); }
+    // z sweep (3D), faces kb..ke over active rows and columns: cal_flux.cpp:140-157.  The PLM states
+    // come from ReconstructYZPLM(prim, ..., k, j) (recon_piecewise_linear.cpp:34-55), whose slope is
+    // taken along y: the face between planes k-1 and k takes cell (k-1, j) plus half its y slope on
+    // the left and cell (k, j) minus half its y slope on the right.  First order copies the cells
+    // (recon_first_order.cpp:18-29).  HLLC rotation for VLZ: normal vz, tangents (vx, vy).
+    void SweepZ() {
+        const int rotate[NV] = {DEN, VZ, VX, VY, PRE};
+        for (int fk = kb; fk <= ke; fk++) {
+#pragma omp parallel for schedule(static)
+            for (int j = jb; j < je; j++) {
+                Real wl[NV], wr[NV], f[NV];
+                for (int i = ib; i < ie; i++) {
+                    for (int q = 0; q < NV; q++) {
+                        int n = rotate[q];
+                        if (P.rc_order == 1) {
+                            wl[q] = CellVal(n, j, i, fk - 1);
+                            wr[q] = CellVal(n, j, i, fk);
+                        } else {
+                            const Real l0 = CellVal(n, j - 1, i, fk - 1), l1 = CellVal(n, j, i, fk - 1),
+                                       l2 = CellVal(n, j + 1, i, fk - 1);
+                            const Real r0 = CellVal(n, j - 1, i, fk), r1 = CellVal(n, j, i, fk),
+                                       r2 = CellVal(n, j + 1, i, fk);
+                            wl[q] = l1 + Real(0.5) * VanLeer(l1 - l0, l2 - l1);
+                            wr[q] = r1 - Real(0.5) * VanLeer(r1 - r0, r2 - r1);
+                        }
+                    }
+                    Hllc(wl, wr, f);
+                    for (int q = 0; q < NV; q++) flx3[at3(rotate[q], fk, j, i)] = f[q];
+                }
+            }
+        }
+    }
 
-    // ---- UpdateCons: src/evolution/update_cons.cpp:11-75 (2 coef), 135-205 (3 coef) ---------
+    void CalFlux() {
+        SweepX();
+        SweepY();
+        if (nz > 1) SweepZ();
+    }
+
+    // ---- UpdateCons: src/evolution/update_cons.cpp:11-75 (2 coef), 135-205 (3 coef); the 3D form
+    //      adds the z flux difference last (update_cons.cpp:76-132) -----------------------------
     void UpdateCons(const std::vector<Real>& U0, std::vector<Real>& UT, Real c1, Real c2, Real c3, bool three) {
-        const Real dtdx = dt * dxRec, dtdy = dt * dyRec;
+        const Real dtdx = dt * dxRec, dtdy = dt * dyRec, dtdz = dt * dzRec;
         const Real dmin = (Real)1e-16, pmin = (Real)1e-16;
         const Real gm1 = gamma - 1;
+        for (int k = kb; k < ke; k++) {
 #pragma omp parallel for schedule(static)
-        for (int j = jb; j < je; j++) {
-            for (int i = ib; i < ie; i++) {
-                Real r[NV];
-                for (int n = 0; n < NV; n++) {
-                    const Real dF = (flx1[at1(n, j, i + 1)] - flx1[at1(n, j, i)]) * dtdx +
-                                    (flx2[at2(n, j + 1, i)] - flx2[at2(n, j, i)]) * dtdy;
-                    if (three)
-                        r[n] = c1 * U0[at(n, j, i)] + c2 * UT[at(n, j, i)] - c3 * dF;
-                    else
-                        r[n] = c1 * U0[at(n, j, i)] - c2 * dF;
+            for (int j = jb; j < je; j++) {
+                for (int i = ib; i < ie; i++) {
+                    Real r[NV];
+                    for (int n = 0; n < NV; n++) {
+                        Real dF = (flx1[at1(n, j, i + 1, k)] - flx1[at1(n, j, i, k)]) * dtdx +
+                                  (flx2[at2(n, j + 1, i, k)] - flx2[at2(n, j, i, k)]) * dtdy;
+                        if (nz > 1) dF = dF + (flx3[at3(n, k + 1, j, i)] - flx3[at3(n, k, j, i)]) * dtdz;
+                        if (three)
+                            r[n] = c1 * U0[at(n, j, i, k)] + c2 * UT[at(n, j, i, k)] - c3 * dF;
+                        else
+                            r[n] = c1 * U0[at(n, j, i, k)] - c2 * dF;
+                    }
+                    if (r[DEN] < dmin) {
+                        r[DEN] = dmin; r[MX] = 0.0; r[MY] = 0.0; r[MZ] = 0.0;
+                        r[ENG] = pmin / gm1 + Real(0.0);
+                    } else {
+                        Real inv = Real(1.0) / r[DEN];
+                        Real ke_ = Real(0.5) * inv * (r[MX] * r[MX] + r[MY] * r[MY] + r[MZ] * r[MZ]);
+                        Real press = gm1 * (r[ENG] - ke_);
+                        if (press < pmin) r[ENG] = pmin / gm1 + ke_;
+                    }
+                    for (int n = 0; n < NV; n++) UT[at(n, j, i, k)] = r[n];
                 }
-                if (r[DEN] < dmin) {
-                    r[DEN] = dmin; r[MX] = 0.0; r[MY] = 0.0; r[MZ] = 0.0;
-                    r[ENG] = pmin / gm1 + Real(0.0);
-                } else {
-                    Real inv = Real(1.0) / r[DEN];
-                    Real ke = Real(0.5) * inv * (r[MX] * r[MX] + r[MY] * r[MY] + r[MZ] * r[MZ]);
-                    Real press = gm1 * (r[ENG] - ke);
-                    if (press < pmin) r[ENG] = pmin / gm1 + ke;
-                }
-                for (int n = 0; n < NV; n++) UT[at(n, j, i)] = r[n];
             }
         }
     }
@@ -549,11 +631,13 @@ struct Oracle {
         for (int a = 0; a < 3; a++) {
             if (g[a] == 0) continue;
             const int m = MX + a;
+            for (int k = kb; k < ke; k++) {
 #pragma omp parallel for schedule(static)
-            for (int j = jb; j < je; j++) {
-                for (int i = ib; i < ie; i++) {
-                    U[at(ENG, j, i)] += U[at(m, j, i)] * g[a] * h;
-                    U[at(m, j, i)] += U[at(DEN, j, i)] * g[a] * h;
+                for (int j = jb; j < je; j++) {
+                    for (int i = ib; i < ie; i++) {
+                        U[at(ENG, j, i, k)] += U[at(m, j, i, k)] * g[a] * h;
+                        U[at(m, j, i, k)] += U[at(DEN, j, i, k)] * g[a] * h;
+                    }
                 }
             }
         }
@@ -580,24 +664,27 @@ struct Oracle {
     void PointGravity(std::vector<Real>& U, Real h) {
         const Real gm = obj0.gm, x0 = obj0.x, y0 = obj0.y, z0 = obj0.z, rs = obj0.rs;
         const Real gmdt = gm * h;
+        for (int k = kb; k < ke; k++) {
+        const Real zk = nz > 1 ? zc[k] : zc0;
 #pragma omp parallel for schedule(static)
         for (int j = jb; j < je; j++) {
             for (int i = ib; i < ie; i++) {
-                const Real delx = xc[i] - x0, dely = yc[j] - y0, delz = zc0 - z0;
+                const Real delx = xc[i] - x0, dely = yc[j] - y0, delz = zk - z0;
                 const Real r2 = delx * delx + dely * dely + delz * delz;
                 const Real rSmoInv = Real(1.0) / std::sqrt(r2 + rs * rs);
                 const Real gVecdtdr = -gmdt * rSmoInv * rSmoInv * rSmoInv;
                 const Real gxdt = gVecdtdr * delx, gydt = gVecdtdr * dely, gzdt = gVecdtdr * delz;
-                Real& d = U[at(DEN, j, i)];
-                Real& mx = U[at(MX, j, i)];
-                Real& my = U[at(MY, j, i)];
-                Real& mz = U[at(MZ, j, i)];
-                Real& e = U[at(ENG, j, i)];
+                Real& d = U[at(DEN, j, i, k)];
+                Real& mx = U[at(MX, j, i, k)];
+                Real& my = U[at(MY, j, i, k)];
+                Real& mz = U[at(MZ, j, i, k)];
+                Real& e = U[at(ENG, j, i, k)];
                 e += mx * gxdt + my * gydt + mz * gzdt;
                 mx += d * gxdt;
                 my += d * gydt;
                 mz += d * gzdt;
             }
+        }
         }
     }
     void BinaryGravity(std::vector<Real>& U, Real tsrc, Real h) {
@@ -607,11 +694,13 @@ struct Oracle {
         obj1.y = ab_ * std::sin(theta);
         const Real x0 = obj0.x, y0 = obj0.y, z0 = obj0.z, x1 = obj1.x, y1 = obj1.y, z1 = obj1.z;
         const Real gIndirdtdr1 = 0;   // the indirect term is computed and then zeroed (binary_gravity.cpp:44-45)
+        for (int k = kb; k < ke; k++) {
+        const Real zk = nz > 1 ? zc[k] : zc0;
 #pragma omp parallel for schedule(static)
         for (int j = jb; j < je; j++) {
             for (int i = ib; i < ie; i++) {
-                const Real delx0 = xc[i] - x0, dely0 = yc[j] - y0, delz0 = zc0 - z0;
-                const Real delx1 = xc[i] - x1, dely1 = yc[j] - y1, delz1 = zc0 - z1;
+                const Real delx0 = xc[i] - x0, dely0 = yc[j] - y0, delz0 = zk - z0;
+                const Real delx1 = xc[i] - x1, dely1 = yc[j] - y1, delz1 = zk - z1;
                 const Real r0Sqr = delx0 * delx0 + dely0 * dely0 + delz0 * delz0;
                 const Real r0SmoInv = Real(1.0) / std::sqrt(r0Sqr + rs0 * rs0);
                 const Real g0 = -gm0dt * r0SmoInv * r0SmoInv * r0SmoInv;
@@ -621,16 +710,17 @@ struct Oracle {
                 const Real gxdt = g0 * delx0 + g1 * delx1 + gIndirdtdr1 * x1;
                 const Real gydt = g0 * dely0 + g1 * dely1 + gIndirdtdr1 * y1;
                 const Real gzdt = g0 * delz0 + g1 * delz1 + gIndirdtdr1 * z1;
-                Real& d = U[at(DEN, j, i)];
-                Real& mx = U[at(MX, j, i)];
-                Real& my = U[at(MY, j, i)];
-                Real& mz = U[at(MZ, j, i)];
-                Real& e = U[at(ENG, j, i)];
+                Real& d = U[at(DEN, j, i, k)];
+                Real& mx = U[at(MX, j, i, k)];
+                Real& my = U[at(MY, j, i, k)];
+                Real& mz = U[at(MZ, j, i, k)];
+                Real& e = U[at(ENG, j, i, k)];
                 e += mx * gxdt + my * gydt + mz * gzdt;
                 mx += d * gxdt;
                 my += d * gydt;
                 mz += d * gzdt;
             }
+        }
         }
     }
     // SourceTerm::UpdateSource: src/source_term/source.cpp:12-21

@@ -30,9 +30,9 @@
  * tests/golden/).  A second build with the reference's own fast flags serves
  * as the timed CPU baseline.
  *
- * State layout: Real[5][leny][lenx] with x fastest -- the reference TArray
- * layout (src/template_array.hpp:67-72) restricted to its single active z
- * plane (the reference keeps 2*nGhost empty z planes in 2D).
+ * State layout: Real[5][lenz][leny][lenx] with x fastest -- the reference TArray
+ * layout (src/template_array.hpp:67-72); in 2D (nz == 1) only the single active
+ * z plane is kept (lenz = 1; the reference keeps 2*nGhost empty z planes).
  */
 #pragma once
 #include <stdint.h>
@@ -48,10 +48,12 @@ typedef struct {
     int integrator;            /* 1 forward Euler, 2 RK2, 3 RK3 (src/sim.cpp:60-66) */
     int rc_order;              /* 1 first order, 2 PLM */
     int stepmax;               /* <= 0: unlimited */
-    int bc[4];                 /* xleft, xright, yleft, yright: 0 copy, 1 periodic, 2 reflective, 3 outflow, 4 fixed */
+    int bc[6];                 /* xleft, xright, yleft, yright, zleft, zright: 0 copy, 1 periodic, 2 reflective,
+                                  3 outflow, 4 fixed (z faces only when nz > 1) */
     double g[3];               /* constant gravity vector; zero vector = not enrolled */
     int npoint;                /* point masses enrolled (0, 1: point gravity, 2: binary) */
     double pg[2][8];           /* per mass: gm, x, y, z, vx, vy, vz, rs (EnrollPointGravity order) */
+    int nz;                    /* active z planes (<= 1: 2D, a single plane) */
 } gko_params;
 
 typedef struct gko_sim gko_sim;
@@ -59,7 +61,7 @@ typedef struct gko_sim gko_sim;
 /* precision: 64 or 32 (32 has no reference counterpart: every operation in float). */
 gko_sim* gko_create(const gko_params* p, int precision);
 void     gko_destroy(gko_sim* s);
-/* Copy the full ghosted state in/out (Real[5][leny][lenx], Real per precision). */
+/* Copy the full ghosted state in/out (Real[5][lenz][leny][lenx], Real per precision). */
 void     gko_set_cons(gko_sim* s, const void* cons);
 void     gko_get_cons(const gko_sim* s, void* cons);
 /* Sim::Advance(dtoutput).  Returns the number of steps taken. */

@@ -5,7 +5,7 @@
 // produce golden fixtures.  It loads the active-region conserved state from the binary
 // file named by $GK_IC_FILE, so every implementation starts from bit-identical initial
 // conditions (the reference setups derive theirs from a truncated pi, src/gaukuk.hpp:19):
-//   int32 nvar(=5), int32 nx, int32 ny, float64 cons[5][ny][nx]
+//   int32 nvar(=5), int32 nx, int32 ny, float64 cons[5][nz][ny][nx] (nz = NZ, 1 in 2D)
 // Optional config keys gx, gy, gz enroll constant gravity
 // (SourceTerm::EnrollConstGravityVector, src/source_term/source.cpp:23-31); npoint (1 or 2) and
 // pg<q>_gm, _x, _y, _z, _vx, _vy, _vz, _rs enroll point masses in order
@@ -82,11 +82,10 @@ void Sim::Setup() {
     if (std::fread(hdr, sizeof(int), 3, f) != 3) throw std::runtime_error("short IC header");
     if (hdr[0] != NVar || hdr[1] != grid.nx || hdr[2] != grid.ny)
         throw std::runtime_error("IC file does not match NX/NY");
-    const size_t n = (size_t)NVar * grid.nx * grid.ny;
+    const size_t n = (size_t)NVar * grid.nx * grid.ny * grid.nz;
     std::vector<double> buf(n);
     if (std::fread(buf.data(), sizeof(double), n, f) != n) throw std::runtime_error("short IC data");
     std::fclose(f);
-    const int k = grid.kb;
     const int fixedbd = (int)Config::getInstance().get("fixedbd", 0.0);
     if (fixedbd == 2) ReadFixedPlane(cons, grid);
     if (fixedbd == 1) {
@@ -98,9 +97,11 @@ void Sim::Setup() {
         }
     }
     for (int v = 0; v < NVar; v++)
-        for (int j = 0; j < grid.ny; j++)
-            for (int i = 0; i < grid.nx; i++)
-                cons(v, k, grid.jb + j, grid.ib + i) = (Real)buf[((size_t)v * grid.ny + j) * grid.nx + i];
+        for (int k = 0; k < grid.nz; k++)
+            for (int j = 0; j < grid.ny; j++)
+                for (int i = 0; i < grid.nx; i++)
+                    cons(v, grid.kb + k, grid.jb + j, grid.ib + i) =
+                        (Real)buf[(((size_t)v * grid.nz + k) * grid.ny + j) * grid.nx + i];
     Real gx = Config::getInstance().get("gx", 0.0);
     Real gy = Config::getInstance().get("gy", 0.0);
     Real gz = Config::getInstance().get("gz", 0.0);

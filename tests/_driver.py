@@ -20,7 +20,7 @@ def driver(setup):
 
 def write_ic(path, cons):
     with open(path, "wb") as f:
-        np.array([5, cons.shape[2], cons.shape[1]], np.int32).tofile(f)
+        np.array([5, cons.shape[-1], cons.shape[-2]], np.int32).tofile(f)
         np.ascontiguousarray(cons, np.float64).tofile(f)
 
 

@@ -23,6 +23,7 @@ def load_run(name):
         gamma=m["gamma"], cfl=m["CFL"], integrator=int(m["integrator"]), rc_order=int(m["RcOrder"]),
         stepmax=int(m["stepmax"]), tmax=m["tmax"], dtoutput=m["dtoutput"],
         bc=(int(m["xleft"]), int(m["xright"]), int(m["yleft"]), int(m["yright"])),
+        nz=int(m.get("NZ", 1)), bcz=(int(m.get("zleft", 1)), int(m.get("zright", 1))),
         g=(m["gx"], m["gy"], m["gz"]), ng=int(m["NGhost"]), cons=z["ic"].copy(),
         point=tuple(tuple(m[f"pg{q}_{k}"] for k in ("gm", "x", "y", "z", "vx", "vy", "vz", "rs"))
                     for q in range(int(m.get("npoint", 0)))),

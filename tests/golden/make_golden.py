@@ -92,6 +92,17 @@ def cases():
     rho, mx, my, mz, e = sh.cons
     sh.cons = P.prim_to_cons(rho, mx / rho, 0.3 + 0 * rho, 0 * rho, 0.8 + 0 * rho, sh.gamma)
     out["shellinner72x60_rk3"] = sh
+    # 3D: periodic shear layer (RK3), a blast with mixed faces under gravity (RK3), outflow/reflect
+    # with point gravity (RK2), type-0 z faces (left unwritten by the reference, so the z ghost
+    # planes keep their initial contents), first order with forward Euler
+    out["shear3d_24x16_rk3"] = P.shear3d(24, 16, seed=3, stepmax=15)
+    out["blast3d_mixed_rk3_g"] = P.blast3d(20, 18, 16, bc=(3, 2, 1, 1), bcz=(2, 3), stepmax=15,
+                                           center=(0.4, 0.5, 0.45)).with_(integrator=3, g=(0.0, -0.1, 0.05))
+    out["blast3d_point_rk2"] = P.blast3d(18, 16, 20, bc=(3, 3, 2, 3), bcz=(3, 2), stepmax=15).with_(
+        point=((0.2, 0.45, 0.4, 0.55, 0, 0, 0, 0.1),))
+    out["blast3d_z0_rk2"] = P.blast3d(16, 20, 14, bc=(1, 1, 3, 3), bcz=(0, 0), stepmax=15, center=(0.4, 0.5, 0.3))
+    out["blast3d_fe_rc1"] = P.blast3d(20, 16, 12, bc=(1, 3, 2, 1), bcz=(1, 1), stepmax=10).with_(
+        integrator=1, rc_order=1)
     return out
 
 
