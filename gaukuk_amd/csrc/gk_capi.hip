@@ -23,6 +23,7 @@ namespace {
 thread_local std::string g_create_error;
 
 struct StageSpec {
+    int q;              // index of the stage in the step
     int in, base, out;  // buffer indices 0 (A = U^n), 1 (B), 2 (C)
     bool three;
     double c1, c2, c3;
@@ -36,8 +37,9 @@ struct gk_ctx {
     gk_phys phys;
     int prec = 64;
     size_t esz = 8;
-    int nx = 0, ny = 0, pitch = 0;
-    long long plane = 0;
+    int nx = 0, ny = 0, nz = 1, pitch = 0;
+    long long plane = 0;     // elements per state variable (all planes)
+    long long plane2 = 0;    // elements per z plane
     void* buf[3] = {nullptr, nullptr, nullptr};
     DevClock* clock = nullptr;
     StepRecord* log = nullptr;
@@ -51,7 +53,7 @@ struct gk_ctx {
     bool uploaded = false;
     bool gravity = false;
     double dxRec = 0, dyRec = 0, drmin = 0;
-    int bc[4] = {0, 0, 0, 0};
+    int bc[6] = {0, 0, 0, 0, 0, 0};
     std::vector<StageSpec> stages;
     std::string err;
     // point masses (gk_set_point_gravity)
@@ -61,6 +63,11 @@ struct gk_ctx {
     double* xc = nullptr;              // device cell centres, ghost-inclusive
     double* yc = nullptr;
     double zc = 0;
+    double* zcs = nullptr;             // 3D: device z centres, ghost-inclusive
+    // 3D: the nearest z ghost plane on each side as the uploaded state holds it (rows -1..ny),
+    // [2][5][ny + 2][nx]; zero when the upload carried no ghost cells
+    void* zstale = nullptr;
+    int stale_base = 0;
     // fixed (type 4) faces (gk_set_fixed_boundary): the saved state in the reference layout, and
     // on the device the two ghost layers nearest each face (xl, xr, yl, yr) in the state precision
     std::vector<double> fixed_ref;
@@ -155,15 +162,16 @@ static void launch_immersed(gk_ctx* c, const StageSpec& s) {
 static void plan_waves(gk_ctx* c, int seg_rows) {
     c->strips = (c->nx + kStrip - 1) / kStrip;
     if (seg_rows <= 0) {
-        const int target = 3072;  // 3 waves per SIMD on 256 CUs x 4 SIMDs: one resident round
-        int segs = (target + c->strips - 1) / c->strips;
+        // one resident round: 3 waves per SIMD on 256 CUs x 4 SIMDs (2 for the 3D variant)
+        const int target = c->nz > 1 ? 2048 : 3072;
+        int segs = (target + c->strips * c->nz - 1) / (c->strips * c->nz);
         seg_rows = (c->ny + segs - 1) / segs;
         if (seg_rows < 8) seg_rows = 8;
     }
     if (seg_rows > c->ny) seg_rows = c->ny;
     c->seg_rows = seg_rows;
     c->segs = (c->ny + seg_rows - 1) / seg_rows;
-    c->nwaves = c->strips * c->segs;
+    c->nwaves = c->strips * c->segs * c->nz;
     int nb = (c->nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
     c->nblocks = (nb + 7) / 8 * 8;  // a multiple of 8 keeps the XCD remap a bijection
 }
@@ -172,15 +180,15 @@ static void build_integrator(gk_ctx* c) {
     c->stages.clear();
     const int it = c->phys.integrator;
     if (it == 1) {  // ForwardEuler_: src/evolution/forward_euler.cpp:11-17
-        c->stages.push_back({0, 0, 1, false, 1.0, 1.0, 0.0, true, true});
+        c->stages.push_back({0, 0, 0, 1, false, 1.0, 1.0, 0.0, true, true});
     } else if (it == 3) {  // RK3_: src/evolution/rk3.cpp:11-28
         const double f13 = 1.0 / 3.0;
-        c->stages.push_back({0, 0, 1, false, 1.0, 1.0, 0.0, true, false});
-        c->stages.push_back({1, 0, 2, true, 0.75, 0.25, 0.25, false, false});
-        c->stages.push_back({2, 0, 0, true, f13, 2 * f13, 2 * f13, false, true});
+        c->stages.push_back({0, 0, 0, 1, false, 1.0, 1.0, 0.0, true, false});
+        c->stages.push_back({1, 1, 0, 2, true, 0.75, 0.25, 0.25, false, false});
+        c->stages.push_back({2, 2, 0, 0, true, f13, 2 * f13, 2 * f13, false, true});
     } else {  // RK2_ (also the reference's default branch): src/evolution/rk2.cpp:11-22
-        c->stages.push_back({0, 0, 1, false, 1.0, 1.0, 0.0, true, false});
-        c->stages.push_back({1, 0, 0, true, 0.5, 0.5, 0.5, false, true});
+        c->stages.push_back({0, 0, 0, 1, false, 1.0, 1.0, 0.0, true, false});
+        c->stages.push_back({1, 1, 0, 0, true, 0.5, 0.5, 0.5, false, true});
     }
 }
 
@@ -193,8 +201,12 @@ static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
     a.out = (R*)c->buf[s.out];
     a.plane = c->plane;
     a.pitch = c->pitch; a.nx = c->nx; a.ny = c->ny;
+    a.nz = c->nz; a.plane2 = c->plane2;
+    a.dzRec = c->nz > 1 ? 1. / ((c->grid.zmax - c->grid.zmin) / c->grid.nz) : 0.0;   // src/sim.cpp:26-33
+    a.zstale = (const R*)c->zstale; a.stale_base = c->stale_base; a.stage_q = s.q;
+    a.zcs = c->zcs;
     a.seg_rows = c->seg_rows; a.strips = c->strips; a.nwaves = c->nwaves;
-    for (int k = 0; k < 4; k++) a.bc[k] = c->bc[k];
+    for (int k = 0; k < 6; k++) a.bc[k] = c->bc[k];
     // RK3's last coefficients are frac13 = 1.0/3.0 and 2*frac13 (src/evolution/rk3.cpp:25-26);
     // rounding the double values to float equals evaluating them in float.
     a.c1 = R(s.c1); a.c2 = R(s.c2); a.c3 = R(s.c3);
@@ -222,22 +234,23 @@ static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
 
 using PlmLimiter = VanLeer;
 
-template <class R, class L, bool THREE, bool FIXED>
+template <class R, class L, bool THREE, int VAR>
 static void launch_stage_f(gk_ctx* c, const StageArgs<R>& a, bool last) {
     dim3 grid(c->nblocks), block(kBlock);
     if (c->gravity) {
-        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, true, FIXED>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, false, FIXED>), grid, block, 0, c->stream, a);
+        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, true, VAR>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, false, VAR>), grid, block, 0, c->stream, a);
     } else {
-        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, true, FIXED>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, false, FIXED>), grid, block, 0, c->stream, a);
+        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, true, VAR>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, false, VAR>), grid, block, 0, c->stream, a);
     }
 }
 
 template <class R, class L, bool THREE>
 static void launch_stage_t(gk_ctx* c, const StageArgs<R>& a, bool last) {
-    if (has_fixed_face(c) || c->inner) launch_stage_f<R, L, THREE, true>(c, a, last);
-    else launch_stage_f<R, L, THREE, false>(c, a, last);
+    if (c->nz > 1) launch_stage_f<R, L, THREE, 2>(c, a, last);
+    else if (has_fixed_face(c) || c->inner) launch_stage_f<R, L, THREE, 1>(c, a, last);
+    else launch_stage_f<R, L, THREE, 0>(c, a, last);
 }
 
 template <class R>
@@ -296,65 +309,87 @@ static gk_status refresh_cmax(gk_ctx* c) {
     dim3 grid(1024), block(kBlock);
     if (c->prec == 32)
         hipLaunchKernelGGL(cmax_kernel<float>, grid, block, 0, c->stream, (const float*)u, c->plane, c->pitch, c->nx,
-                           c->ny, EosPolicy<float>::make(c->phys.gamma), c->clock);
+                           c->ny * c->nz, EosPolicy<float>::make(c->phys.gamma), c->clock);
     else
         hipLaunchKernelGGL(cmax_kernel<double>, grid, block, 0, c->stream, (const double*)u, c->plane, c->pitch, c->nx,
-                           c->ny, EosPolicy<double>::make(c->phys.gamma), c->clock);
+                           c->ny * c->nz, EosPolicy<double>::make(c->phys.gamma), c->clock);
     GK_TRY_HIP(c, hipGetLastError());
     return GK_OK;
 }
 
 static int current_buffer(gk_ctx* c) { return c->hclock->cur ? 1 : 0; }
 
-// Reference layout helpers: Real[5][lenz][leny][lenx], active plane k = nghost.
+// Reference layout helpers: Real[5][lenz][leny][lenx]; active planes k = nghost .. nghost + nz - 1
+// (one plane in 2D).  Active states here are [5][nz][ny][nx].
 template <class R>
 static void ref_to_active(const gk_ctx* c, const R* ref, double* act) {
-    const int g = c->grid.nghost, lx = c->nx + 2 * g, ly = c->ny + 2 * g, lz = 1 + 2 * g;
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, nz = c->nz;
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = nz + 2 * g;
     for (int n = 0; n < 5; n++)
-        for (int j = 0; j < c->ny; j++)
-            for (int i = 0; i < c->nx; i++)
-                act[((size_t)n * c->ny + j) * c->nx + i] =
-                    (double)ref[(((size_t)n * lz + g) * ly + (j + g)) * lx + (i + g)];
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++)
+                    act[(((size_t)n * nz + k) * ny + j) * nx + i] =
+                        (double)ref[(((size_t)n * lz + g + k) * ly + (j + g)) * lx + (i + g)];
 }
 
 template <class R>
 static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
-    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
-    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = 1 + 2 * g;
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, nz = c->nz;
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = nz + 2 * g;
     std::memset(ref, 0, sizeof(R) * 5 * (size_t)lx * ly * lz);
-    auto at = [&](int n, int j, int i) -> R& { return ref[(((size_t)n * lz + g) * ly + j) * lx + i]; };
+    auto at = [&](int n, int k, int j, int i) -> R& { return ref[(((size_t)n * lz + k) * ly + j) * lx + i]; };
     for (int n = 0; n < 5; n++)
-        for (int j = 0; j < ny; j++)
-            for (int i = 0; i < nx; i++) at(n, j + g, i + g) = (R)act[((size_t)n * ny + j) * nx + i];
-    // ghost fill over the active tangential range, x faces then y faces (src/boundary/boundary.cpp:127-142);
-    // fixed faces copy the saved state at the same position (src/boundary/fixed.cpp:16-123)
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) at(n, k + g, j + g, i + g) = (R)act[(((size_t)n * nz + k) * ny + j) * nx + i];
+    // ghost fill over the active tangential range, x faces, y faces, then z faces
+    // (src/boundary/boundary.cpp:127-142); fixed faces copy the saved state at the same position
+    // (src/boundary/fixed.cpp:16-123); type-0 z faces leave their planes zero (the reference's z copy
+    // refills the y ghosts instead, boundary.cpp:90-122)
     auto saved = [&](int n, int j, int i) {
-        return (R)c->fixed_ref[(((size_t)n * lz + g) * ly + j) * lx + i];
+        return (R)c->fixed_ref[(((size_t)n * (1 + 2 * g) + g) * ly + j) * lx + i];
     };
-    for (int j = g; j < g + ny; j++)
-        for (int i = 0; i < lx; i++) {
-            if (i >= g && i < g + nx) continue;
-            int kind;
-            const int si = bc_remap(i - g, nx, c->bc[0], c->bc[1], kind) + g;
-            R u[5];
-            for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, j, si);
-            bc_transform(u, 1, kind);
-            for (int n = 0; n < 5; n++) at(n, j, i) = u[n];
-        }
-    if (ny > 1)
-        for (int j = 0; j < ly; j++) {
-            if (j >= g && j < g + ny) continue;
-            for (int i = g; i < g + nx; i++) {
+    for (int k = g; k < g + nz; k++) {
+        for (int j = g; j < g + ny; j++)
+            for (int i = 0; i < lx; i++) {
+                if (i >= g && i < g + nx) continue;
                 int kind;
-                const int sj = bc_remap(j - g, ny, c->bc[2], c->bc[3], kind) + g;
+                const int si = bc_remap(i - g, nx, c->bc[0], c->bc[1], kind) + g;
                 R u[5];
-                for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, sj, i);
-                bc_transform(u, 2, kind);
-                for (int n = 0; n < 5; n++) at(n, j, i) = u[n];
+                for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, k, j, si);
+                bc_transform(u, 1, kind);
+                for (int n = 0; n < 5; n++) at(n, k, j, i) = u[n];
             }
+        if (ny > 1 || nz > 1)
+            for (int j = 0; j < ly; j++) {
+                if (j >= g && j < g + ny) continue;
+                for (int i = g; i < g + nx; i++) {
+                    int kind;
+                    const int sj = bc_remap(j - g, ny, c->bc[2], c->bc[3], kind) + g;
+                    R u[5];
+                    for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, k, sj, i);
+                    bc_transform(u, 2, kind);
+                    for (int n = 0; n < 5; n++) at(n, k, j, i) = u[n];
+                }
+            }
+    }
+    if (nz > 1)
+        for (int k = 0; k < lz; k++) {
+            if (k >= g && k < g + nz) continue;
+            const int zt = k < g ? c->bc[4] : c->bc[5];
+            if (zt < 1 || zt > 3) continue;
+            int kind;
+            const int sk = bc_remap(k - g, nz, c->bc[4], c->bc[5], kind) + g;
+            for (int j = g; j < g + ny; j++)
+                for (int i = g; i < g + nx; i++) {
+                    R u[5];
+                    for (int n = 0; n < 5; n++) u[n] = at(n, sk, j, i);
+                    bc_transform(u, 3, kind);
+                    for (int n = 0; n < 5; n++) at(n, k, j, i) = u[n];
+                }
         }
 }
-
 
 template <class T>
 static gk_status upload_vec(gk_ctx* c, T** dst, const std::vector<T>& v) {
@@ -420,21 +455,28 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
     if (grid->nx <= 0 || grid->ny <= 0 || grid->nz <= 0 || grid->nghost < 0) {
         g_create_error = "Invalid grid parameters from config"; return GK_EINVAL;
     }
-    if (grid->nz != 1) { g_create_error = "only 2D/1D grids (NZ = 1) run on this path"; return GK_EINVAL; }
+    if (grid->nz > 1) {
+        // 3D: face types 0-3 on all six faces (fixed faces, the inner region, immersed shells,
+        // ensembles and the learned correction are 2D only)
+        for (int f = 0; f < 6; f++)
+            if (phys->bc[f] == 4) { g_create_error = "3D grids take face types 0-3"; return GK_EINVAL; }
+        if (grid->ny < 2) { g_create_error = "3D grids need NY >= 2"; return GK_EINVAL; }
+    }
     if (grid->nghost < 2) { g_create_error = "NGhost must be >= 2 for the PLM stencil"; return GK_EINVAL; }
     if (grid->nx < 2) { g_create_error = "NX must be >= 2"; return GK_EINVAL; }
     if (phys->precision != 64 && phys->precision != 32) { g_create_error = "precision must be 64 or 32"; return GK_EINVAL; }
     if (!(phys->gamma > 1.0)) { g_create_error = "gamma must exceed 1"; return GK_EINVAL; }
-    if ((long long)grid->nx * grid->ny * 5 >= (1LL << 40)) { g_create_error = "grid too large"; return GK_EINVAL; }
+    if ((long long)grid->nx * grid->ny * grid->nz * 5 >= (1LL << 40)) { g_create_error = "grid too large"; return GK_EINVAL; }
 
     gk_ctx* c = new gk_ctx();
     c->grid = *grid;
     c->phys = *phys;
     c->prec = phys->precision;
     c->esz = c->prec == 32 ? 4 : 8;
-    c->nx = grid->nx; c->ny = grid->ny; c->pitch = grid->nx;
-    c->plane = (long long)c->pitch * c->ny;
-    for (int f = 0; f < 4; f++) c->bc[f] = (phys->bc[f] >= 0 && phys->bc[f] <= 4) ? phys->bc[f] : 0;
+    c->nx = grid->nx; c->ny = grid->ny; c->nz = grid->nz > 1 ? grid->nz : 1; c->pitch = grid->nx;
+    c->plane2 = (long long)c->pitch * c->ny;
+    c->plane = c->plane2 * c->nz;
+    for (int f = 0; f < 6; f++) c->bc[f] = (phys->bc[f] >= 0 && phys->bc[f] <= 4) ? phys->bc[f] : 0;
     if (c->ny == 1) { c->bc[2] = 0; c->bc[3] = 0; }  // y fluxes are multiplied by dyRec = 0
     c->gravity = phys->g[0] != 0 || phys->g[1] != 0 || phys->g[2] != 0;
     // Domain (src/sim.cpp:16-36)
@@ -506,9 +548,19 @@ void gk_destroy(gk_ctx* c) {
     if (c->sh_tar) (void)hipFree(c->sh_tar);
     if (c->sh_dir) (void)hipFree(c->sh_dir);
     if (c->sh_gam) (void)hipFree(c->sh_gam);
+    if (c->zcs) (void)hipFree(c->zcs);
+    if (c->zstale) (void)hipFree(c->zstale);
     if (c->hclock) (void)hipHostFree(c->hclock);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+// 3D: the cells of the nearest z ghost planes the reference never writes keep the uploaded
+// state's values in the array that holds it (zero in the other): [2][5][ny + 2][nx], rows -1..ny.
+static gk_status set_stale(gk_ctx* c, const std::vector<double>& planes) {
+    if (c->nz <= 1) return GK_OK;
+    c->stale_base = c->hclock->step;
+    return upload_real(c, &c->zstale, planes);
 }
 
 gk_status gk_upload_active(gk_ctx* c, const double* host) {
@@ -526,6 +578,7 @@ gk_status gk_upload_active(gk_ctx* c, const double* host) {
         GK_TRY_HIP(c, hipMemcpy(c->buf[0], host, n * 8, hipMemcpyHostToDevice));
     }
     c->uploaded = true;
+    if ((s = set_stale(c, std::vector<double>((size_t)10 * (c->ny + 2) * c->nx, 0.0))) != GK_OK) return s;
     return refresh_cmax(c);
 }
 
@@ -550,7 +603,20 @@ gk_status gk_upload_cons(gk_ctx* c, const void* host) {
     std::vector<double> act((size_t)c->plane * 5);
     if (c->prec == 32) ref_to_active(c, (const float*)host, act.data());
     else ref_to_active(c, (const double*)host, act.data());
-    return gk_upload_active(c, act.data());
+    gk_status s = gk_upload_active(c, act.data());
+    if (s != GK_OK || c->nz <= 1) return s;
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, nz = c->nz;
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = nz + 2 * g;
+    std::vector<double> planes((size_t)10 * (ny + 2) * nx);
+    for (int side = 0; side < 2; side++)
+        for (int n = 0; n < 5; n++)
+            for (int jj = -1; jj <= ny; jj++)
+                for (int i = 0; i < nx; i++) {
+                    const size_t src = (((size_t)n * lz + (side ? g + nz : g - 1)) * ly + (g + jj)) * lx + (g + i);
+                    const double v = c->prec == 32 ? (double)((const float*)host)[src] : ((const double*)host)[src];
+                    planes[(((size_t)side * 5 + n) * (ny + 2) + (jj + 1)) * nx + i] = v;
+                }
+    return set_stale(c, planes);
 }
 
 gk_status gk_download_cons(gk_ctx* c, void* host) {
@@ -627,6 +693,14 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
         std::vector<double> hx, hy;
         domain_centres(c, hx, hy, c->zc);
         const int lx = (int)hx.size(), ly = (int)hy.size();
+        if (c->nz > 1) {
+            // every plane's z centre (Domain, src/sim.cpp:39)
+            const int g = c->grid.nghost;
+            const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
+            std::vector<double> hz((size_t)c->nz + 2 * g);
+            for (int k = 0; k < (int)hz.size(); k++) hz[k] = c->grid.zmin - g * dz + 0.5 * dz + k * dz;
+            if (gk_status s = upload_vec(c, &c->zcs, hz)) return s;
+        }
         GK_TRY_HIP(c, hipMalloc(&c->xc, sizeof(double) * lx));
         GK_TRY_HIP(c, hipMalloc(&c->yc, sizeof(double) * ly));
         GK_TRY_HIP(c, hipMemcpy(c->xc, hx.data(), sizeof(double) * lx, hipMemcpyHostToDevice));
@@ -647,6 +721,7 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
 
 gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host, int fix_inner) {
     if (!c || !host) return GK_EINVAL;
+    if (c->nz > 1) return fail(c, GK_EINVAL, "fixed faces and the inner region are 2D only");
     // Boundary::EnrollFixedBoundaryData: the ghost layers of the given state become the fixed faces'
     // values; the stencil reads the two layers nearest each face
     const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
@@ -688,6 +763,7 @@ gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, doub
     if (!c) return GK_EINVAL;
     // Boundary::SetupImmersedShell (src/boundary/immersed.cpp:89-235), nz == 1
     if (c->nx <= 1 || c->ny <= 1) return fail(c, GK_EINVAL, "immersed boundary has to be at least 2D. ");
+    if (c->nz > 1) return fail(c, GK_EINVAL, "immersed shells are 2D only");
     const int g = c->grid.nghost, ib = g, ie = g + c->nx, jb = g, je = g + c->ny;
     std::vector<double> xc, yc;
     double zc;
@@ -891,10 +967,10 @@ gk_status gk_cons_sums(gk_ctx* c, double out[5]) {
     const void* u = c->buf[current_buffer(c)];
     if (c->prec == 32)
         hipLaunchKernelGGL(sums_partial_kernel<float>, dim3(c->sum_blocks), dim3(256), 0, c->stream, (const float*)u,
-                           c->plane, c->pitch, c->nx, c->ny, c->partial);
+                           c->plane, c->pitch, c->nx, c->ny * c->nz, c->partial);
     else
         hipLaunchKernelGGL(sums_partial_kernel<double>, dim3(c->sum_blocks), dim3(256), 0, c->stream, (const double*)u,
-                           c->plane, c->pitch, c->nx, c->ny, c->partial);
+                           c->plane, c->pitch, c->nx, c->ny * c->nz, c->partial);
     hipLaunchKernelGGL(sums_final_kernel, dim3(1), dim3(64), 0, c->stream, c->partial, c->sum_blocks, c->sums);
     GK_TRY_HIP(c, hipGetLastError());
     GK_TRY_HIP(c, hipMemcpyAsync(out, c->sums, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));

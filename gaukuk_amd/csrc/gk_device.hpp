@@ -47,7 +47,7 @@ struct StageArgs {
     int pitch, nx, ny;
     int seg_rows;       // rows per wave segment
     int strips, nwaves; // wave grid: strips x segments
-    int bc[4];          // xleft, xright, yleft, yright (ny == 1: y forced to type 0)
+    int bc[6];          // xleft, xright, yleft, yright, zleft, zright (2D, ny == 1: y forced to type 0)
     R c1, c2, c3;       // combine coefficients
     int kick_in;        // stage input is U^n: apply the first gravity half-kick on load
     int kick_base;      // three-coefficient base is U^n: apply the first half-kick on load
@@ -72,6 +72,17 @@ struct StageArgs {
     // and per active row the column interval [x, y) of region cells (empty when x >= y)
     const R* inner;
     const int2* inner_rows;
+    // 3D (nz > 1): planes of ny rows follow each other in every state plane of `plane` elements.
+    // zstale: the nearest z ghost plane on each side (rows -1..ny) of the array the reference's
+    // Sim holds as `cons` at step stale_base, [2][5][ny + 2][nx]; the reference never writes those
+    // cells (edge cells, and whole planes of type-0 z faces), so they keep that array's initial
+    // contents, and the other array's are zero.  stage_q: the stage's index in the step.
+    int nz;
+    long long plane2;            // elements per z plane (ny * pitch)
+    double dzRec;
+    const R* zstale;
+    int stale_base, stage_q;
+    const double* zcs;           // z centres, ghost-inclusive index
     const double* xc;            // cell centres, ghost-inclusive index (Domain, src/sim.cpp:37-39)
     const double* yc;
     double zc;                   // centre of the active z plane

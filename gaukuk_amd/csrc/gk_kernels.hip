@@ -42,14 +42,16 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 #endif
 
 // GRAV: constant gravity is enrolled; LAST: the stage ends the step (second half-kick, next
-// step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.  FIXED: a
-// face has type 4 (its ghost cells come from saved layers); a separate variant so that grids
-// without such faces run code with no trace of them.
+// step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.  VAR selects
+// the boundary/geometry variant, so that plain 2D grids run code with no trace of the others:
+// 0 plain 2D; 1 2D with fixed faces or the inner fixed region (saved data); 2 3D (nz > 1).
 // One wave advances the cells of strip `strip`, segment `seg` of one grid by one stage.
 // `strip` and `seg` are wave-uniform (scalar registers), so row logic is scalar branches and
 // scalar address arithmetic.
-template <class R, class Lim, bool THREE, bool GRAV, bool LAST, bool FIXED>
-__device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane) {
+template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR>
+__device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane, int kp = 0) {
+    constexpr bool FIXED = VAR == 1;
+    constexpr bool THREED = VAR == 2;
     constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
     constexpr bool KICK_BASE = GRAV && THREE; // the combine base is U^n
     DevClock* clk = a.clock;
@@ -73,17 +75,30 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             by1[q] = R(a.ab * sin(th));
         }
     }
-    auto kick = [&](R u[5], int q, int i, int j) {
+    // the z centre of plane kz (3D) or of the single plane (2D); the 3D table exists only when
+    // point masses are enrolled, the only source that reads it
+    auto zcentre = [&](int kz) -> R { return (THREED && a.npoint) ? R(a.zcs[kz + a.ng]) : R(a.zc); };
+    const R zc_here = zcentre(kp);
+    auto kick_at = [&](R u[5], int q, int i, int j, R z) {
         gravity_kick(u, a.gx, a.gy, a.gz, h);
         if (a.npoint) point_kick(u, a.npoint, a.pm0, a.pm1, bx1[q], by1[q], h, R(a.xc[i + a.ng]),
-                                 R(a.yc[j + a.ng]), R(a.zc));
+                                 R(a.yc[j + a.ng]), z);
     };
+    auto kick = [&](R u[5], int q, int i, int j) { kick_at(u, q, i, j, zc_here); };
     const long long P = a.plane;
     const int pitch = a.pitch;
     // forward-Euler ping-pong: the current state alternates between the two buffers
     const R* in = a.in;
     R* out = a.out;
     if (a.pingpong && clk->cur) { in = a.out; out = const_cast<R*>(a.in); }
+    // 3D: the wave works in plane kp; its rows are addressed from the plane's first row
+    const R* in3 = in;        // plane 0 of the stage input, for the z neighbours
+    const R* base = a.base;
+    if (THREED) {
+        in += kp * a.plane2;
+        out += kp * a.plane2;
+        base += kp * a.plane2;
+    }
 
     int kx;
     const int cs = bc_remap(min(c, a.nx + 1), a.nx, a.bc[0], a.bc[1], kx);
@@ -192,13 +207,88 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         for (int n = 0; n < 5; n++) Fn[n] = __shfl_down(F[n], 1);
     };
 
+    // ---- z faces (3D) ----------------------------------------------------------------------
+    // The reference's z sweep reconstructs with ReconstructYZPLM at (k, j), whose slope runs along y
+    // (recon_piecewise_linear.cpp:34-55, cal_flux.cpp:140-157): the face between planes k-1 and k
+    // takes W(k-1, j) + sigma_y/2 on the left and W(k, j) - sigma_y/2 on the right (first order:
+    // the cells themselves).  This wave has both states of its own plane from the y march; the
+    // planes below and above are read directly (their rows come from L2: neighbouring planes are
+    // marched by neighbouring waves).
+    const R dtdz = R(dt * a.dzRec);
+    R wRz[5];   // W - sigma_y/2 of the row the next update reads (plane kp)
+    // The stage reads the array that held the state at step stale_base on even steps from there
+    // when it is the first stage (the reference's `cons`), on odd steps otherwise (`consTemp`,
+    // swapped with `cons` at every step end).
+    const bool readsX = THREED && ((a.stage_q == 0) == (((clk->step - a.stale_base) & 1) == 0));
+    // raw conserved values of (plane kz, row jz) in this lane's column, boundary rules applied
+    auto zload = [&](int kz, int jz, R u[5]) {
+        if (kz >= 0 && kz < a.nz) {
+            int ky;
+            const int rs = bc_remap(jz, a.ny, a.bc[2], a.bc[3], ky);
+            const R* row = in3 + kz * a.plane2 + (long long)rs * pitch;
+#pragma unroll
+            for (int n = 0; n < 5; n++) u[n] = at(row, n, off_in);
+            if (KICK_IN) kick_at(u, 0, cs, rs, zcentre(kz));
+            if (jz < 0 || jz >= a.ny) bc_transform(u, 2, ky);
+            return;
+        }
+        // a z ghost plane: z faces of types 1-3 fill its active rows (periodic.cpp:135-190,
+        // reflective.cpp:170-230, outflow.cpp:216-300); type 0 registers the y copy instead
+        // (boundary.cpp:90-122), and no face fills edge cells
+        const int zt = kz < 0 ? a.bc[4] : a.bc[5];
+        if ((zt == 1 || zt == 2 || zt == 3) && jz >= 0 && jz < a.ny) {
+            int kk;
+            const int ks = bc_remap(kz, a.nz, a.bc[4], a.bc[5], kk);
+            const R* row = in3 + ks * a.plane2 + (long long)jz * pitch;
+#pragma unroll
+            for (int n = 0; n < 5; n++) u[n] = at(row, n, off_in);
+            if (KICK_IN) kick_at(u, 0, cs, jz, zcentre(ks));
+            bc_transform(u, 3, kk);
+        } else {
+            const long long pl = (long long)(a.ny + 2) * a.nx;
+            const long long o = (long long)(kz < 0 ? 0 : 5) * pl + (long long)(jz + 1) * a.nx + cown;
+#pragma unroll
+            for (int n = 0; n < 5; n++) u[n] = readsX ? a.zstale[o + n * pl] : R(0);
+        }
+    };
+    // W -+ sigma_y/2 of (plane kz, row jz)
+    auto zstate = [&](int kz, int jz, R lo[5], R hi[5]) {
+        R u0[5], u1[5], u2[5], w0[5], w1[5], w2[5];
+        zload(kz, jz - 1, u0);
+        zload(kz, jz, u1);
+        zload(kz, jz + 1, u2);
+        a.eos.cons_to_prim(u0, w0);
+        a.eos.cons_to_prim(u1, w1);
+        a.eos.cons_to_prim(u2, w2);
+        yslope(w0, w1, w2, lo, hi);
+    };
+    // z-face flux with the direction-VLZ rotation (hllc.cpp:25-27): (den, vz, vx, vy, p)
+    auto zflux = [&](const R wl[5], const R wr[5], R H[5]) {
+        const R l[5] = {wl[0], wl[3], wl[1], wl[2], wl[4]};
+        const R rr[5] = {wr[0], wr[3], wr[1], wr[2], wr[4]};
+        R f[5];
+        RiemannPolicy::flux(a.eos, l, rr, f);
+        H[0] = f[0]; H[3] = f[1]; H[1] = f[2]; H[2] = f[3]; H[4] = f[4];
+    };
+    // the z-flux difference H(k+1/2) - H(k-1/2) of row rr, from this plane's states (lo, hi)
+    auto zfluxes = [&](int rr, const R lo_here[5], const R hi_here[5], R Hz[5]) {
+        R lo[5], hi[5], Hd[5], Hu[5];
+        zstate(kp - 1, rr, lo, hi);
+        zflux(hi, lo_here, Hd);
+        zstate(kp + 1, rr, lo, hi);
+        zflux(hi_here, lo, Hu);
+#pragma unroll
+        for (int n = 0; n < 5; n++) Hz[n] = Hu[n] - Hd[n];
+    };
+
     R pre[5];               // next input row, raw
     R wLy[5], Gp[5];        // left state of the next y face; y flux at the lower face of row r-2
     R cacc = R(0);
     const int rlast = j1 + 1;
 
     // the update of row r-2 from its x fluxes (F, Fn) and y fluxes (Gp below, G above)
-    auto update = [&](int r, const R bs[5], const R ut[5], const R F[5], const R Fn[5], const R G[5]) {
+    auto update = [&](int r, const R bs[5], const R ut[5], const R F[5], const R Fn[5], const R G[5],
+                      const R Hz[5]) {
         const long long o = (long long)(r - 2) * pitch;
         R b2[5];
 #pragma unroll
@@ -209,7 +299,8 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         R res[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-            const R dF = (Fn[n] - F[n]) * dtdx + (G[n] - Gp[n]) * dtdy;
+            R dF = (Fn[n] - F[n]) * dtdx + (G[n] - Gp[n]) * dtdy;
+            if (THREED) dF = dF + Hz[n] * dtdz;   // update_cons.cpp:76-132: the z term is added last
             if (THREE) res[n] = a.c1 * b2[n] + a.c2 * ut[n] - a.c3 * dF;
             else res[n] = a.c1 * b2[n] - a.c2 * dF;
         }
@@ -230,7 +321,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         const long long o = (long long)(r - 2) * pitch;
         if (THREE) {
 #pragma unroll
-            for (int n = 0; n < 5; n++) { bs[n] = at(a.base + o, n, off_own); ut[n] = at(in + o, n, off_own); }
+            for (int n = 0; n < 5; n++) { bs[n] = at(base + o, n, off_own); ut[n] = at(in + o, n, off_own); }
         } else {
 #pragma unroll
             for (int n = 0; n < 5; n++) { bs[n] = at(in + o, n, off_own); ut[n] = R(0); }
@@ -265,6 +356,10 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         finish_row(j0 + 1, u, W0);     // W0 now holds row j0+1
         R wLn[5];
         yslope(W1, W2, W0, wR, wLn);   // slope of row j0
+        if (THREED) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) wRz[n] = wR[n];
+        }
         yflux(wLy, wR, Gp);            // face j0
 #pragma unroll
         for (int n = 0; n < 5; n++) wLy[n] = wLn[n];
@@ -281,9 +376,11 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         yslope(A, B, Cn, wR, wLn);
         yflux(wLy, wR, G);
         xflux(A, F, Fn);
-        update(r, bs, ut, F, Fn, G);
+        R Hz[5] = {R(0), R(0), R(0), R(0), R(0)};
+        if (THREED) zfluxes(r - 2, wRz, wLy, Hz);
+        update(r, bs, ut, F, Fn, G, Hz);
 #pragma unroll
-        for (int n = 0; n < 5; n++) { Gp[n] = G[n]; wLy[n] = wLn[n]; }
+        for (int n = 0; n < 5; n++) { Gp[n] = G[n]; wLy[n] = wLn[n]; wRz[n] = wR[n]; }
     };
     for (int r = j0 + 2;;) {
         body(r, W2, W0, W1); if (++r > rlast) break;
@@ -299,15 +396,23 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 
 // One RK stage of one grid: wave w of the (XCD-remapped) grid takes strip w % strips, segment
 // w / strips.
-template <class R, class Lim, bool THREE, bool GRAV, bool LAST, bool FIXED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GK_WAVES_PER_EU)))
+// 3D waves hold the z-face work on top of the 2D march: two waves per SIMD keep it out of scratch.
+template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VAR == 2 ? 2 : GK_WAVES_PER_EU)))
 void stage_kernel(const StageArgs<R> a) {
     if (!a.clock->active) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
     if (w >= a.nwaves) return;
-    march<R, Lim, THREE, GRAV, LAST, FIXED>(a, w % a.strips, w / a.strips, lane);
+    if (VAR == 2) {
+        // 3D: strip fastest, then segment, then plane (waves of neighbouring planes share an XCD)
+        const int per_plane = a.nwaves / a.nz;
+        const int kp = w / per_plane, r = w - kp * per_plane;
+        march<R, Lim, THREE, GRAV, LAST, VAR>(a, r % a.strips, r / a.strips, lane, kp);
+    } else {
+        march<R, Lim, THREE, GRAV, LAST, VAR>(a, w % a.strips, w / a.strips, lane);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -433,7 +538,7 @@ void ens_stage_kernel(const EnsArgs<R> e) {
             a.dxRec = d.dxRec; a.dyRec = d.dyRec;
             a.clock = clk;
             const int local = item - d.item_base;
-            march<R, Lim, THREE, GRAV, LAST, false>(a, local % d.strips, local / d.strips, lane);
+            march<R, Lim, THREE, GRAV, LAST, 0>(a, local % d.strips, local / d.strips, lane);
         }
         item = next;
     }

@@ -1259,6 +1259,7 @@ gk_status gk_flux_record(gk_ctx* c, int on) {
     if (c->gravity) return fail(c, GK_EINVAL, "flux recording needs a grid without gravity");
     if (has_extra_boundaries(c))
         return fail(c, GK_EINVAL, "flux recording takes face types 0-3 only (no fixed, inner or immersed boundaries)");
+    if (c->nz > 1) return fail(c, GK_EINVAL, "flux recording is 2D only");
     if (c->ny < 2) return fail(c, GK_EINVAL, "flux recording needs a 2D grid");
     const long long nf = face_count(c);
     if (!c->acc) GK_TRY_HIP(c, hipMalloc(&c->acc, sizeof(double) * 4 * (size_t)nf));
@@ -1389,6 +1390,7 @@ gk_status gk_set_net(gk_ctx* c, gk_net* n) {
     if (c->gravity) return fail(c, GK_EINVAL, "the corrected stage does not take gravity");
     if (has_extra_boundaries(c))
         return fail(c, GK_EINVAL, "the corrected stage takes face types 0-3 only (no fixed, inner or immersed boundaries)");
+    if (c->nz > 1) return fail(c, GK_EINVAL, "the corrected stage is 2D only");
     if (c->phys.rc_order == 1) return fail(c, GK_EINVAL, "the corrected stage uses PLM (RcOrder 2)");
     if (c->nx < kLT || c->ny < kLT) return fail(c, GK_EINVAL, "the corrected stage needs NX, NY >= 16");
     if (!c->lstats) {

@@ -179,7 +179,7 @@ def _dp(a):
 
 def make_grid(prob) -> GkGrid:
     g = GkGrid()
-    g.nx, g.ny, g.nz, g.nghost = prob.nx, prob.ny, 1, prob.ng
+    g.nx, g.ny, g.nz, g.nghost = prob.nx, prob.ny, prob.nz, prob.ng
     (g.xmin, g.xmax, g.ymin, g.ymax, g.zmin, g.zmax) = prob.extent
     return g
 
@@ -192,7 +192,7 @@ def make_phys(prob, precision: int = 64) -> GkPhys:
     p.integrator, p.rc_order, p.stepmax = prob.integrator, prob.rc_order, prob.stepmax
     for k in range(4):
         p.bc[k] = prob.device_bc()[k]
-    p.bc[4] = p.bc[5] = 1
+    p.bc[4], p.bc[5] = prob.bcz
     p.precision = precision
     return p
 
@@ -240,9 +240,13 @@ class Solver:
         except Exception:
             pass
 
+    def _active_shape(self):
+        p = self.prob
+        return (5, p.nz, p.ny, p.nx) if p.nz > 1 else (5, p.ny, p.nx)
+
     def upload(self, cons: np.ndarray):
         a = np.ascontiguousarray(cons, np.float64)
-        assert a.shape == (5, self.prob.ny, self.prob.nx)
+        assert a.shape == self._active_shape()
         self._check(self.L.gk_upload_active(self.h, _dp(a)))
         self._bd_done = False
 
@@ -253,13 +257,13 @@ class Solver:
             self.update_bd()
 
     def download(self) -> np.ndarray:
-        out = np.empty((5, self.prob.ny, self.prob.nx), np.float64)
+        out = np.empty(self._active_shape(), np.float64)
         self._check(self.L.gk_download_active(self.h, _dp(out)))
         return out
 
     def _ref_shape(self):
         ng = self.prob.ng
-        return (5, 1 + 2 * ng, self.prob.ny + 2 * ng, self.prob.nx + 2 * ng)
+        return (5, max(self.prob.nz, 1) + 2 * ng, self.prob.ny + 2 * ng, self.prob.nx + 2 * ng)
 
     def download_ref_layout(self) -> np.ndarray:
         dt = np.float64 if self.precision == 64 else np.float32

@@ -86,8 +86,11 @@ Boundary::Boundary() {
     Bdxr = ForType(0, true, type[1], fixed_);
     Bdyl = ForType(1, false, type[2], fixed_);
     Bdyr = ForType(1, true, type[3], fixed_);
-    Bdzl = ForType(2, false, type[4], fixed_);
-    Bdzr = ForType(2, true, type[5], fixed_);
+    // z faces of type 0 (and unknown codes) register the y copy functions, as the reference does
+    // (src/boundary/boundary.cpp:90-122): their ghost planes are never written
+    const bool zl = type[4] >= 1 && type[4] <= 4, zr = type[5] >= 1 && type[5] <= 4;
+    Bdzl = zl ? ForType(2, false, type[4], fixed_) : ForType(1, false, 0, fixed_);
+    Bdzr = zr ? ForType(2, true, type[5], fixed_) : ForType(1, true, 0, fixed_);
 }
 
 void Boundary::EnrollFixedBoundaryData(const TArray<Real>& cons, const Grid&, bool fixInner) {

@@ -228,20 +228,25 @@ SETUP_SIZES = {
     "shock_tube": dict(NX=128, NY=4, NZ=1, stepmax=20),
     "solid_body": dict(NX=96, NY=96, stepmax=20),
     "wave_test": dict(NX=64, NY=64, stepmax=20),
+    # the shipped 3D inputs at reduced 3D sizes
+    "blast3d": dict(setup="blast", NX=32, NY=32, NZ=32, stepmax=10),
+    "shock_tube3d": dict(setup="shock_tube", NX=128, NY=4, NZ=4, stepmax=20),
 }
 
 
 def reference_setups():
     """Each reference setup file, built into the reference itself, on its shipped input file."""
     for name, over in SETUP_SIZES.items():
-        cfg = Config.load(os.path.join("/root/reference/input", f"{name}.in"))
+        over = dict(over)
+        setup = over.pop("setup", name)
+        cfg = Config.load(os.path.join("/root/reference/input", f"{setup}.in"))
         vals = dict(cfg.data)
         vals.update(over)
         vals["tmax"] = 1e30
         vals["dtoutput"] = 1e30
         with tempfile.TemporaryDirectory() as d:
             write_in(os.path.join(d, "case.in"), vals)
-            r = subprocess.run([os.path.join(REF, f"gaukuk_ref_{name}"), "-i", "case.in"], cwd=d,
+            r = subprocess.run([os.path.join(REF, f"gaukuk_ref_{setup}"), "-i", "case.in"], cwd=d,
                                env=dict(os.environ, OMP_NUM_THREADS="4"), capture_output=True, text=True,
                                check=True)
             steps = int(re.search(r"total steps =\s*(\d+)", r.stdout).group(1))
@@ -249,9 +254,10 @@ def reference_setups():
             outs = sorted(f for f in os.listdir(d) if f.startswith(kind + "_"))
             last = GaukukOutput(os.path.join(d, outs[-1]))
             np.savez_compressed(os.path.join(HERE, f"setup_{name}.npz"), final=last.active().copy(),
-                                t=np.array(float(last.t)), steps=np.array(steps), kind=kind,
+                                t=np.array(float(last.t)), steps=np.array(steps), kind=kind, setup=setup,
                                 meta=json.dumps(vals))
-        print(f"setup {name:14s} {vals['NX']}x{vals['NY']} steps={steps} t={float(last.t)!r} ({kind})")
+        print(f"setup {name:14s} {vals['NX']}x{vals['NY']}x{vals.get('NZ', 1)} steps={steps} "
+              f"t={float(last.t)!r} ({kind})")
 
 
 def main():

@@ -33,6 +33,11 @@ def test_create_rejects_bad_grid_without_gpu():
     prob = P.kelvin_helmholtz(8, seed=0)
     g = hip.make_grid(prob)
     g.nz = 2
+    phys = hip.make_phys(prob)
+    phys.bc[4] = 4   # fixed faces are 2D only: rejected before any device work
     h = ctypes.c_void_p()
+    st = hip.lib().gk_create(ctypes.byref(g), ctypes.byref(phys), ctypes.byref(h))
+    assert st == 1 and b"3D grids take face types 0-3" in hip.lib().gk_last_error(None)
+    g.nz, g.nghost = 1, 1
     st = hip.lib().gk_create(ctypes.byref(g), ctypes.byref(hip.make_phys(prob)), ctypes.byref(h))
-    assert st == 1 and b"NZ" in hip.lib().gk_last_error(None)
+    assert st == 1 and b"NGhost" in hip.lib().gk_last_error(None)

@@ -27,7 +27,8 @@ def rel_l1(a, b):
 @pytest.mark.parametrize("name", ["kh64_rk3", "rt32x96_rk2_g", "blast48x56_mixed", "sod128_fe_rc1",
                                   "point48_rk2_bc3", "binary40x44_rk3_g", "fixed48x40_rk3",
                                   "fixedvar52x44_rk2_g", "solid96_rk2", "inner64_rk3_g",
-                                  "shellinner72x60_rk3"])
+                                  "shellinner72x60_rk3", "blast3d_mixed_rk3_g", "blast3d_z0_rk2",
+                                  "blast3d_point_rk2"])
 def test_driver_golden(tmp_path, name):
     prob, final, t, steps = load_run(name)
     print(f"driver case {name}", flush=True)

@@ -2,7 +2,8 @@
 
 Each setup file of the reference (src/setup/setup_XXX.cpp) is compiled unchanged against this host
 driver (`make -C gaukuk_amd/host external SETUP_SRC=...`, done by the build where the reference
-sources are present) and run on the reference's shipped input file at a reduced 2D size.  The
+sources are present) and run on the reference's shipped input file at a reduced size (2D, and the
+3D inputs blast.in and shock_tube.in also in 3D).  The
 fixtures (tests/golden/setup_XXX.npz) come from the reference built with the same setup file and
 the same input values; results must agree at the FP64 parity bar (1e-12 relative L1 per variable)
 with identical step counts.  The setups exercise periodic, reflective, outflow and fixed faces,
@@ -23,7 +24,8 @@ from tests._fixtures import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-SETUPS = ["blast", "disk", "disk_planet", "kh", "rt", "shock_tube", "solid_body", "wave_test"]
+SETUPS = ["blast", "disk", "disk_planet", "kh", "rt", "shock_tube", "solid_body", "wave_test", "blast3d",
+          "shock_tube3d"]
 
 
 def rel_l1(a, b):
@@ -33,10 +35,10 @@ def rel_l1(a, b):
 
 @pytest.mark.parametrize("name", SETUPS)
 def test_reference_setup_file(tmp_path, name):
-    binary = os.path.join(HOST, "bin", f"gaukuk_external_{name}.sim")
+    z = np.load(os.path.join(GOLDEN, f"setup_{name}.npz"), allow_pickle=False)
+    binary = os.path.join(HOST, "bin", f"gaukuk_external_{str(z['setup'])}.sim")
     if not os.path.exists(binary):
         pytest.skip(f"{binary} not built (the reference setup files were not available at build time)")
-    z = np.load(os.path.join(GOLDEN, f"setup_{name}.npz"), allow_pickle=False)
     vals = json.loads(str(z["meta"]))
     kind = str(z["kind"])
     write_in(str(tmp_path / "case.in"), vals)
