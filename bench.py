@@ -148,6 +148,41 @@ def bench_rt(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_3d(args):
+    """The 3D path: a periodic 3D shear layer at 256^3, FP64, PLM+HLLC+RK3 (no BASELINE metric;
+    zone-updates/s and the stage kernels' HBM fraction for the design notes)."""
+    from gaukuk_amd import hip
+    from gaukuk_amd import problems as P
+    n = 256
+    prob = P.shear3d(n, n, seed=0, stepmax=-1)
+    s = hip.Solver(prob, seg_rows=args.seg_rows)
+    log(f"3D shear {n}^3, library {hip.version()}, tile {s.tile()}")
+    s.run_steps(args.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    total_ms, kms, cnt = s.profile(args.steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    assert s.check_finite()
+    zones = n ** 3
+    kernels = []
+    for k, bpc in enumerate((80.0, 120.0, 120.0)):
+        avg_ms = kms[k] / max(cnt[k], 1)
+        kernels.append({"kernel": f"stage_kernel<double, VanLeer, ..., VAR=2> (RK3 stage {k + 1})",
+                        "avg_launch_us": avg_ms * 1e3, "bytes_per_launch": bpc * zones,
+                        "achieved_GBs": bpc * zones / (avg_ms * 1e-3) / 1e9})
+    dom = max(kernels, key=lambda q: q["avg_launch_us"])
+    out = {"metric": "zone-updates/s (3D path)", "value": zones * args.steps / wall, "unit": "zone-updates/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"3D shear layer {n}^3 FP64, PLM+HLLC+RK3, CFL 0.4, periodic"},
+           "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": dom["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom["kernel"],
+                        "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": dom["avg_launch_us"]},
+           "stage_kernels": kernels, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def bench_learned(args):
     """Learned flux correction on coarse KH: the corrected FP32 256^2 solver (the trained network
     gaukuk_amd/data/kh256_flux_mlp.npz) against plain FP64 at 256^2 and 512^2 and plain FP32 at
@@ -215,7 +250,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seg-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "rt", "learned"],
+    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "rt", "learned", "3d"],
                     help="kh: the headline (default); ensemble / rt / learned: the other benchmark configurations")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -230,6 +265,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "rt":
         return bench_rt(args)
+    if args.workload == "3d":
+        return bench_3d(args)
     if args.workload == "learned":
         return bench_learned(args)
 
