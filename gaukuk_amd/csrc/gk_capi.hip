@@ -664,6 +664,17 @@ gk_status gk_get_time(gk_ctx* c, double* t, double* dt, int* step, int* is_conti
     return GK_OK;
 }
 
+gk_status gk_set_time(gk_ctx* c, double t, int step) {
+    if (!c || step < 0) return GK_EINVAL;
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
+    c->hclock->t = t;
+    c->hclock->step = step;
+    c->hclock->is_continue = !(c->hclock->stepmax > 0 && step >= c->hclock->stepmax);
+    c->stale_base = step;
+    return write_clock(c);
+}
+
 gk_status gk_step_log(gk_ctx* c, int* steps, double* t, double* dt, int cap, int* count) {
     if (!c || cap < 0) return GK_EINVAL;
     gk_status s = read_clock(c);

@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get("GK_LIB_PATH") or os.path.join(HERE, "libgaukuk_hip.so
 GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ESTATE", 5: "GK_ENUMERIC"}
 
 # Every symbol include/gaukuk.h declares (checked by tests/test_capi_symbols.py).
-SYMBOLS = ["gk_set_point_gravity", "gk_set_fixed_boundary", "gk_set_immersed_shell", "gk_update_bd", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
+SYMBOLS = ["gk_set_time", "gk_set_point_gravity", "gk_set_fixed_boundary", "gk_set_immersed_shell", "gk_update_bd", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
            "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version",
@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
         L.gk_set_fixed_boundary.argtypes = [vp, vp, ip]
         L.gk_set_immersed_shell.restype = st
         L.gk_set_immersed_shell.argtypes = [vp] + [C.c_double] * 4
+        L.gk_set_time.restype = st
+        L.gk_set_time.argtypes = [vp, C.c_double, ip]
         L.gk_update_bd.restype = st
         L.gk_update_bd.argtypes = [vp]
         for name in ("gk_upload_cons", "gk_download_cons"):

@@ -1,7 +1,8 @@
 // main.cpp -- `gaukuk.sim -i input.in`: the reference's driver loop (src/main.cpp:12-65):
 // load the configuration, build the Sim, run the setup plug-in, fill boundaries, write output 0,
 // then Advance by dtoutput and write one output per interval until tmax (or stepmax).
-// Extra flags: --setup-only (stop after output 0), --quiet (no per-step lines).
+// Extra flags: --setup-only (stop after output 0), --quiet (no per-step lines), --restart cons_XXXXX
+// (continue from that output file of the same grid: its state and time; outputs resume at XXXXX+1).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -16,13 +17,16 @@
 int main(int argc, char* argv[]) {
     using namespace Gaukuk;
     if (argc < 3 || std::string(argv[1]) != "-i") {
-        std::cerr << "Usage: " << argv[0] << " -i input.in [--setup-only] [--quiet]" << std::endl;
+        std::cerr << "Usage: " << argv[0] << " -i input.in [--setup-only] [--quiet] [--restart cons_XXXXX]"
+                  << std::endl;
         return 1;
     }
     bool setupOnly = false, quiet = false;
+    const char* restart = nullptr;
     for (int a = 3; a < argc; a++) {
         if (!std::strcmp(argv[a], "--setup-only")) setupOnly = true;
         else if (!std::strcmp(argv[a], "--quiet")) quiet = true;
+        else if (!std::strcmp(argv[a], "--restart") && a + 1 < argc) restart = argv[++a];
     }
     try {
         Config::getInstance().loadFromFile(argv[2]);
@@ -32,14 +36,18 @@ int main(int argc, char* argv[]) {
         Sim sim;
         sim.SetVerbose(!quiet);
         sim.Setup();
-        sim.boundary.UpdateBD(sim.cons, sim.grid, sim.domain, sim.eos);
-        sim.MarkHostModified();
-
         DataType outputType = DataType::Cons;
         if ((int)Config::getInstance().get("dataType") == 1) outputType = DataType::Prim;
         int outputStep = 0;
+        if (restart) {
+            // the file holds the state as the run left it: no boundary update, no output 0
+            outputStep = sim.Restart(restart);
+        } else {
+            sim.boundary.UpdateBD(sim.cons, sim.grid, sim.domain, sim.eos);
+            sim.MarkHostModified();
+            sim.WriteData(outputStep, outputType);
+        }
         Real tnow = sim.GetTime();
-        sim.WriteData(outputStep, outputType);
         if (setupOnly) return 0;
 
         auto time0 = std::chrono::high_resolution_clock::now();

@@ -1,7 +1,9 @@
 // sim.cpp -- host driver: construction from the configuration, device set-up and Advance.
 #include "sim.hpp"
 
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 #include <vector>
@@ -140,12 +142,43 @@ void Sim::EnsureDevice() {
         Check(gk_upload_cons(ctx_, cons.data()), "gk_upload_cons");
         hostDirty_ = false;
     }
+    if (timePending_) {
+        Check(gk_set_time(ctx_, t, step), "gk_set_time");
+        timePending_ = false;
+    }
     // the in-place part of the last boundary update (immersed shell, inner region)
     if (boundary.DeviceUpdatePending()) {
         Check(gk_update_bd(ctx_), "gk_update_bd");
         boundary.ClearDeviceUpdate();
         hostStale_ = true;
     }
+}
+
+int Sim::Restart(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open restart file " + path);
+    int realSize = 0, hdr[5] = {0, 0, 0, 0, 0};
+    Real tFile = 0;
+    in.read(reinterpret_cast<char*>(&realSize), sizeof(int));
+    in.read(reinterpret_cast<char*>(&tFile), sizeof(Real));
+    in.read(reinterpret_cast<char*>(hdr), sizeof hdr);
+    if (!in || realSize != (int)sizeof(Real) || hdr[0] != NVar || hdr[1] != grid.nx || hdr[2] != grid.ny ||
+        hdr[3] != grid.nz || hdr[4] != grid.nGhost)
+        throw std::runtime_error("restart file " + path + " does not match this grid and precision");
+    in.seekg((std::streamoff)(domain.xc.GetSizeInBytes() + domain.yc.GetSizeInBytes() + domain.zc.GetSizeInBytes()),
+             std::ios::cur);
+    int size = 0;
+    in.read(reinterpret_cast<char*>(&size), sizeof(int));
+    if (!in || size != (int)cons.GetSize()) throw std::runtime_error("restart file " + path + ": bad data size");
+    in.read(reinterpret_cast<char*>(cons.data()), cons.GetSizeInBytes());
+    if (!in) throw std::runtime_error("restart file " + path + ": short data");
+    t = tFile;
+    step = 0;
+    hostDirty_ = true;
+    hostStale_ = false;
+    timePending_ = true;
+    const size_t cut = path.find_last_of('_');
+    return cut == std::string::npos ? 0 : std::atoi(path.c_str() + cut + 1);
 }
 
 // Sim::Advance (src/sim.cpp:69-117): the whole loop (dt from the CFL condition, source

@@ -40,6 +40,9 @@ public:
     void Setup();  // provided by the linked setup_XXX.cpp
     void Advance(Real dtoutput);
     void WriteData(const int outputID, DataType dType);
+    // Continue from a cons_XXXXX file of this grid (an extension: the reference always starts at
+    // t = 0): loads its state, ghost cells included, and its time; returns XXXXX.
+    int Restart(const std::string& path);
 
     Real GetTime() { return t; }
     Real Getdt() { return dt; }
@@ -63,6 +66,7 @@ private:
     bool hostDirty_ = true;   // host cons newer than the device copy
     bool hostStale_ = false;  // device newer than host cons
     int fixedSent_ = -1;      // version of the fixed-face data the device holds (-1: none)
+    bool timePending_ = false;  // a restart time the device has not been given yet
     bool shellSent_ = false;  // the immersed shell is set up on the device
     bool selfFaces_[6] = {false, false, false, false, false, false};  // self-defined faces run as fixed
     TArray<Real> selfGhosts_;  // the ghost layers those faces write

@@ -108,6 +108,10 @@ gk_status gk_download_active(gk_ctx* ctx, double* host);
 gk_status gk_advance(gk_ctx* ctx, double dtoutput, double* t, int* step);
 /* Sim::GetTime / Getdt / GetStep and isContinue (src/sim.hpp:41-43, :26). */
 gk_status gk_get_time(gk_ctx* ctx, double* t, double* dt, int* step, int* is_continue);
+/* Set the simulation time and step count (a restart from an output file; the reference has no
+ * restart, its Sim starts at t = 0, src/sim.cpp:45-47).  The next step's dt follows from the current
+ * state as usual. */
+gk_status gk_set_time(gk_ctx* ctx, double t, int step);
 /* The per-step lines the reference prints (src/sim.cpp:100-103): the last n (<= cap) steps'
  * (step, t, dt), oldest first.  *count receives the number written. */
 gk_status gk_step_log(gk_ctx* ctx, int* steps, double* t, double* dt, int cap, int* count);

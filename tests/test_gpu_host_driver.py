@@ -96,3 +96,23 @@ def test_driver_rejects_state_dependent_self_face(tmp_path):
     vals["sdcopy_0"] = 1
     with pytest.raises(RuntimeError, match="depends on the state"):
         run_from_file(str(tmp_path), prob, extra_args=("--quiet",), in_values=vals)
+
+
+@pytest.mark.parametrize("name", ["kh64_rk3", "blast3d_z0_rk2"])
+def test_driver_restart_continues_bitwise(tmp_path, name):
+    """--restart cons_00001 continues a run from its output file: the next output equals the
+    uninterrupted run's bit for bit (2D, and 3D with type-0 z faces)."""
+    prob, _, _, _ = load_run(name)
+    vals = prob.in_values()
+    vals.update(tmax=0.04, dtoutput=0.02, stepmax=-1)
+    full, part = tmp_path / "full", tmp_path / "part"
+    full.mkdir()
+    part.mkdir()
+    print(f"restart {name}", flush=True)
+    run_from_file(str(full), prob, extra_args=("--quiet",), in_values=vals)
+    run_from_file(str(part), prob, extra_args=("--quiet",), in_values=dict(vals, tmax=0.02))
+    assert not (part / "cons_00002").exists()
+    run_from_file(str(part), prob, extra_args=("--quiet", "--restart", "cons_00001"), in_values=vals)
+    a, b = GaukukOutput(str(full / "cons_00002")), GaukukOutput(str(part / "cons_00002"))
+    assert a.t == b.t
+    assert np.array_equal(a.active(), b.active())
