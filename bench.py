@@ -38,27 +38,28 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(prob, budget_s: float = 12.0):
-    """Time the oracle (fast build) on this host: a bounded sample of the same KH workload."""
+def cpu_baseline(prob, budget_s: float = 12.0, label: str = "KH", precision: int = 64):
+    """Time the oracle (fast build) on this host: a bounded sample of the same workload."""
     threads = int(os.environ.get("GK_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     from oracle import gko  # test infrastructure: the CPU baseline only
     # one probe step sizes the sample to about budget_s seconds of stepping
-    s1 = gko.OracleSim(prob.with_(stepmax=1), flavour="fast")
+    s1 = gko.OracleSim(prob.with_(stepmax=1), precision=precision, flavour="fast")
     t0 = time.perf_counter()
     s1.advance(1e30)
     one = time.perf_counter() - t0
     del s1
     n = max(2, min(200, int(budget_s / max(one, 1e-3))))
-    s2 = gko.OracleSim(prob.with_(stepmax=n), flavour="fast")
+    s2 = gko.OracleSim(prob.with_(stepmax=n), precision=precision, flavour="fast")
     t0 = time.perf_counter()
     s2.advance(1e30)
     el = time.perf_counter() - t0
     steps = s2.step
-    zu = prob.nx * prob.ny * steps / el
+    zu = prob.nx * prob.ny * prob.nz * steps / el
+    rk = {1: "Euler", 2: "RK2", 3: "RK3"}[prob.integrator]
     return {"value": zu, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-            "sample": f"KH {prob.nx}x{prob.ny} FP64 RK3, {steps} steps of the oracle restatement of the "
+            "sample": f"{label} {prob.nx}x{prob.ny} FP{precision} {rk}, {steps} steps of the oracle restatement of the "
                       f"reference OpenMP path (-O3 -ffast-math -funroll-loops -fopenmp), {el:.1f} s, "
                       f"setup excluded"}
 
@@ -107,7 +108,27 @@ def bench_ensemble(args):
         "speedup_vs_looped": zu(ms_graph, K) / zu(ms_looped, kl),
         "step_hbm_frac": per_step_bytes / (ms_graph / K * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "wall_s_graph": wall_graph,
+        # the persistent stage launches dominate the step; their bytes are the whole step's
+        "roofline": {"bound": "hbm", "achieved": per_step_bytes / (ms_graph / K * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": per_step_bytes / (ms_graph / K * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "ens_stage_kernel<float, VanLeer, ...> (both RK2 stages, per step)",
+                     "bytes_per_launch": per_step_bytes / 2, "avg_launch_us": ms_graph / K * 1e3 / 2},
     }
+    if not args.no_cpu:
+        # the oracle on a sample of the ensemble: the largest problems, one after another
+        log("CPU baseline (oracle, fast build)")
+        big = sorted(probs, key=lambda p: p.nx * p.ny)[-4:]
+        done, el = 0, 0.0
+        for p in big:
+            cb = cpu_baseline(p, budget_s=3.0, label="blast", precision=32)
+            done += 1
+            el += p.nx * p.ny / cb["value"]
+        cb["value"] = sum(p.nx * p.ny for p in big) / el
+        cb["sample"] = (f"the 4 largest problems of the ensemble, FP32 RK2, about 3 s each of the oracle "
+                        f"restatement of the reference OpenMP path (-O3 -ffast-math -funroll-loops -fopenmp)")
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 
@@ -134,6 +155,23 @@ def bench_rt(args):
     a, b = res[32]["state"], res[64]["state"]
     l1 = [float(np.abs(a[n] - b[n]).sum() / max(np.abs(b[n]).sum(), 1e-300)) for n in range(5)]
     zu = {p: prob.nx * prob.ny * res[p]["steps"] / res[p]["wall"] for p in res}
+    # per-kernel HIP-event timing of the FP32 stage kernels (RK2 with gravity; a state is 20 B per cell
+    # in FP32: stage 1 reads U^n and writes U1 = 40 B/cell, stage 2 reads U1 and U^n and writes = 60 B/cell)
+    s = hip.Solver(prob.with_(tmax=1e30, dtoutput=1e30), precision=32)
+    s.run_steps(5)
+    s.synchronize()
+    _, kms, cnt = s.profile(args.steps)
+    cells = prob.nx * prob.ny
+    kernels = []
+    for k, (name, bpc) in enumerate((("stage_kernel<float, VanLeer, THREE=false, GRAV=true, LAST=false, VAR=0> "
+                                       "(RK2 stage 1)", 40.0),
+                                      ("stage_kernel<float, VanLeer, THREE=true, GRAV=true, LAST=true, VAR=0> "
+                                       "(RK2 stage 2)", 60.0))):
+        avg_ms = kms[k] / max(cnt[k], 1)
+        kernels.append({"kernel": name, "launches": cnt[k], "avg_launch_us": avg_ms * 1e3,
+                        "bytes_per_launch": bpc * cells, "achieved_GBs": bpc * cells / (avg_ms * 1e-3) / 1e9})
+    dom = max(kernels, key=lambda q: q["avg_launch_us"] * q["launches"])
+    log("FP32 stage kernels: " + ", ".join(f"{q['avg_launch_us']:.1f} us" for q in kernels))
     out = {
         "metric": METRIC, "value": zu[64], "unit": "zone-updates/s", "n_gpus": 1,
         "steps": res[64]["steps"], "warmup": args.warmup, "ms_per_step": res[64]["wall"] * 1e3 / res[64]["steps"],
@@ -144,7 +182,16 @@ def bench_rt(args):
         "fp32_vs_fp64_rel_l1": l1,
         "mass_drift_fp64": float(res[64]["sums1"][0] / res[64]["sums0"][0] - 1),
         "mass_drift_fp32": float(res[32]["sums1"][0] / res[32]["sums0"][0] - 1),
+        "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom["kernel"],
+                     "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": dom["avg_launch_us"]},
+        "fp32_stage_kernels": kernels,
     }
+    if not args.no_cpu:
+        log("CPU baseline (oracle, fast build)")
+        out["cpu_baseline"] = cpu_baseline(prob, label="Rayleigh-Taylor", precision=64)
+    else:
+        out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 
