@@ -677,8 +677,19 @@ __device__ __forceinline__ double draw(unsigned long long bits, int emax, bool s
     if (sign && ((bits >> 63) & 1ull)) v |= 1ull << 63;
     return __longlong_as_double((long long)v);
 }
+// FP32 operand: 23 mantissa bits, exponent in [-emax, emax]
+__device__ __forceinline__ float drawf(unsigned mant, unsigned bits, int emax) {
+    const int e = (int)(bits % (unsigned)(2 * emax + 1)) - emax;
+    unsigned v = ((unsigned)(e + 127) << 23) | (mant & 0x7FFFFFu);
+    if ((bits >> 31) & 1u) v |= 1u << 31;
+    return __uint_as_float(v);
+}
+// FP64 draws with exponents up to +-emax; FP32 draws with exponents up to +-min(emax, 60) whose
+// divisor mantissa runs through all 2^23 values as k does (count >= 2^23 covers every divisor
+// mantissa); each mismatch against the IEEE operation counts once.
 __global__ void selftest_kernel(long long n, unsigned long long seed, int emax, unsigned long long* bad) {
     unsigned long long local = 0;
+    const int emax32 = emax < 60 ? emax : 60;
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
         const double a = draw(mix64(seed ^ (3ull * k)), emax, true);
         const double b = draw(mix64(seed ^ (3ull * k + 1)), emax, true);
@@ -687,6 +698,11 @@ __global__ void selftest_kernel(long long n, unsigned long long seed, int emax, 
         const double r_ieee = __builtin_sqrt(x), r = qsqrt(x);
         local += (__double_as_longlong(q) != __double_as_longlong(q_ieee));
         local += (__double_as_longlong(r) != __double_as_longlong(r_ieee));
+        const unsigned long long h = mix64(seed ^ (3ull * k + 5));
+        const float a32 = drawf((unsigned)h, (unsigned)(h >> 32), emax32);
+        const float b32 = drawf((unsigned)k, (unsigned)(h >> 40) * 2654435761u, emax32);
+        const float q32_ieee = a32 / b32, q32 = qdiv(a32, b32);
+        local += (__float_as_uint(q32) != __float_as_uint(q32_ieee));
     }
     if (local) atomicAdd(bad, local);
 }
