@@ -478,6 +478,13 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
     c->plane = c->plane2 * c->nz;
     for (int f = 0; f < 6; f++) c->bc[f] = (phys->bc[f] >= 0 && phys->bc[f] <= 4) ? phys->bc[f] : 0;
     if (c->ny == 1) { c->bc[2] = 0; c->bc[3] = 0; }  // y fluxes are multiplied by dyRec = 0
+    // 3D: a z face of type 0 (or an unknown code) registers the y copy (boundary.cpp:90-114), which
+    // UpdateBD runs after both y faces (boundary.cpp:127-135) over the active x and z range, so the
+    // y ghosts on that side are copies of the nearest row whatever the y face's own type
+    if (c->nz > 1) {
+        if (c->bc[4] == 0) c->bc[2] = 0;
+        if (c->bc[5] == 0) c->bc[3] = 0;
+    }
     c->gravity = phys->g[0] != 0 || phys->g[1] != 0 || phys->g[2] != 0;
     // Domain (src/sim.cpp:16-36)
     const double dx = (grid->xmax - grid->xmin) / grid->nx;

@@ -5,6 +5,8 @@ It produces, next to this script:
   run_<case>.npz   whole-run fixtures: initial conserved state (active cells), the .in values,
                    the reference's final active state, final time and step count
   vec_<op>.npz     per-function vectors from oracle/_ref/ref_vectors (reference objects)
+  random_configs.npz  (--random) step count, final time and checksums (tests/_random_cases.py) of
+                   the reference's final state on every seeded random configuration
   setup_<s>.npz    the reference built with its own setup file s (oracle/_ref/gaukuk_ref_<s>, `make -C
                    oracle ref-setups`) run on its own input file at a reduced size: the .in values,
                    the last output file's data, its type, final time and step count
@@ -260,14 +262,34 @@ def reference_setups():
               f"t={float(last.t)!r} ({kind})")
 
 
+def random_configs():
+    """The reference on every seeded random configuration: steps, final time, state checksums."""
+    from tests._random_cases import CASES, checksums, random_problem
+    steps, ts, sums = [], [], []
+    for seed in range(CASES):
+        prob = random_problem(seed)
+        with tempfile.TemporaryDirectory() as d:
+            final, t, n, _ = run_reference(prob, d)
+        steps.append(n)
+        ts.append(t)
+        sums.append(checksums(final))
+    np.savez_compressed(os.path.join(HERE, "random_configs.npz"), steps=np.array(steps), t=np.array(ts),
+                        sums=np.array(sums))
+    print(f"random configurations: {CASES} cases written")
+
+
 def main():
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default="", help="comma-separated run cases to (re)make; default: everything")
     ap.add_argument("--setups", action="store_true", help="(re)make only the reference-setup fixtures")
+    ap.add_argument("--random", action="store_true", help="(re)make only the random-configuration checksums")
     a = ap.parse_args()
     if a.setups:
         reference_setups()
+        return
+    if a.random:
+        random_configs()
         return
     only = [v for v in a.only.split(",") if v]
     if not os.path.exists(os.path.join(REF, "gaukuk_ref")):

@@ -4,6 +4,8 @@ The fixtures come from oracle/_ref (reference sources + a from-file setup plug-i
 flags); see tests/golden/make_golden.py.  The oracle's strict build must reproduce them bit for
 bit: final state, final time and step count of whole runs, and every per-function vector.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -42,3 +44,19 @@ def test_eos_vectors():
     assert np.array_equal(gko.cons_to_prim(v["cons"], 1.4).view(np.uint64), v["c2p"].view(np.uint64))
     assert np.array_equal(gko.prim_to_cons(v["p2c_in"], 1.4).view(np.uint64), v["p2c"].view(np.uint64))
     assert gko.cmax(v["cons"], 1.4) == float(v["cmax"][0])
+
+
+def test_random_configurations_match_reference():
+    """200 seeded random configurations (1D/2D/3D, every face type, integrator, order, gravity,
+    point masses, inner region, immersed shell): the oracle's step count, final time and state
+    checksums equal the compiled reference's."""
+    from tests._random_cases import CASES, checksums, describe, random_problem
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "random_configs.npz"))
+    assert len(ref["steps"]) == CASES
+    for seed in range(CASES):
+        prob = random_problem(seed)
+        sim = gko.OracleSim(prob).run_to_end()
+        assert sim.step == ref["steps"][seed], describe(seed, prob)
+        assert sim.t == ref["t"][seed], describe(seed, prob)
+        np.testing.assert_allclose(checksums(sim.cons()), ref["sums"][seed], rtol=1e-14, atol=1e-300,
+                                   err_msg=describe(seed, prob))
