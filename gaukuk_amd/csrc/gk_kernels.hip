@@ -15,13 +15,31 @@
 // are computed once per face, so every cell is converted once and every face solved once
 // (plus the strip halo).  Loads of the next row are issued before the current row's flux math.
 // Waves never exchange data, so there is no LDS and no barrier; x neighbours move through
-// cross-lane permutes.
+// DPP lane shifts.
 #include "gk_device.hpp"
 
 namespace gk {
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Neighbour-lane values through DPP wavefront shifts: one VALU move per dword instead of an LDS
+// permute and its wait.  lane_below gives lane i the value of lane i-1 (wave_shr:1), lane_above
+// that of lane i+1 (wave_shl:1); the end lane with no source (0, resp. 63) keeps its own value,
+// as __shfl_up/__shfl_down return it.  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v) {
+    const int b = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xf, 0xf, false));
+}
+template <class R> __device__ __forceinline__ R lane_below(R v) { return dpp_shift<0x138>(v); }
+template <class R> __device__ __forceinline__ R lane_above(R v) { return dpp_shift<0x130>(v); }
 
 // Max over the 64 lanes of a wave (all lanes get the result).
 __device__ __forceinline__ double wave_max(double v) {
@@ -195,16 +213,16 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         R wR[5], wLo[5], wL[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-            const R am = __shfl_up(A[n], 1), ap = __shfl_down(A[n], 1);
+            const R am = lane_below(A[n]), ap = lane_above(A[n]);
             const R s = Lim::slope(A[n] - am, ap - A[n]);
             wR[n] = A[n] - R(0.5) * s;   // right state of face c
             wLo[n] = A[n] + R(0.5) * s;  // left state of face c+1
         }
 #pragma unroll
-        for (int n = 0; n < 5; n++) wL[n] = __shfl_up(wLo[n], 1);
+        for (int n = 0; n < 5; n++) wL[n] = lane_below(wLo[n]);
         RiemannPolicy::flux(a.eos, wL, wR, F);
 #pragma unroll
-        for (int n = 0; n < 5; n++) Fn[n] = __shfl_down(F[n], 1);
+        for (int n = 0; n < 5; n++) Fn[n] = lane_above(F[n]);
     };
 
     // ---- z faces (3D) ----------------------------------------------------------------------
@@ -690,6 +708,16 @@ __device__ __forceinline__ float drawf(unsigned mant, unsigned bits, int emax) {
 __global__ void selftest_kernel(long long n, unsigned long long seed, int emax, unsigned long long* bad) {
     unsigned long long local = 0;
     const int emax32 = emax < 60 ? emax : 60;
+    {
+        // the stage kernel's DPP lane shifts against the cross-lane shuffles (every lane active)
+        const unsigned long long h = mix64(~seed ^ (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x));
+        const double v = draw(h, emax, true);
+        const float f = drawf((unsigned)h, (unsigned)(h >> 32), emax32);
+        local += dbits(lane_below(v)) != dbits(__shfl_up(v, 1));
+        local += dbits(lane_above(v)) != dbits(__shfl_down(v, 1));
+        local += __float_as_uint(lane_below(f)) != __float_as_uint(__shfl_up(f, 1));
+        local += __float_as_uint(lane_above(f)) != __float_as_uint(__shfl_down(f, 1));
+    }
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
         const double a = draw(mix64(seed ^ (3ull * k)), emax, true);
         const double b = draw(mix64(seed ^ (3ull * k + 1)), emax, true);

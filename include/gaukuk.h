@@ -136,7 +136,8 @@ gk_status gk_get_tile(gk_ctx* ctx, int* seg_rows, int* strips, int* waves);
 /* Device self-check, no counterpart in the reference: the short FP64 quotient and square-root
  * sequences of the stage kernels against the IEEE operators on `count` operand draws with
  * binary exponents in [-emax, emax] (root operands [-2*emax, 2*emax]); *mismatches counts results
- * whose bits differ.  Expected 0 for emax <= 500. */
+ * whose bits differ; also the stage kernels' neighbour-lane moves against the cross-lane shuffles
+ * (one draw per launched thread).  Expected 0 for emax <= 500. */
 gk_status gk_selftest(long long count, unsigned long long seed, int emax, long long* mismatches);
 
 /* ---- Ensembles: many independent grids advanced together (no counterpart in the reference,

@@ -141,7 +141,8 @@ def test_download_ref_layout_ghosts(hip):
 @pytest.mark.parametrize("emax", [8, 100, 500])
 def test_short_div_sqrt_bitwise(hip, emax):
     """The stage kernels' FP64 quotient/root and FP32 quotient sequences return the IEEE bits for
-    normal operands (FP32: every divisor mantissa, eight times)."""
+    normal operands (FP32: every divisor mantissa, eight times), and their DPP lane shifts move
+    the same values as the cross-lane shuffles."""
     assert hip.selftest(count=1 << 26, seed=7 + emax, emax=emax) == 0
 
 
