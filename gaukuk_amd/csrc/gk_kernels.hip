@@ -14,8 +14,8 @@
 // consecutive rows; the y-face flux between rows r-2 and r-1 and the x-face fluxes of row r-2
 // are computed once per face, so every cell is converted once and every face solved once
 // (plus the strip halo).  Loads of the next row are issued before the current row's flux math.
-// Waves never exchange data, so there is no LDS and no barrier; x neighbours move through
-// DPP lane shifts.
+// Waves never exchange data, so there is no LDS allocation and no barrier; x neighbours move
+// between lanes (DPP shifts in FP64, permutes in FP32).
 #include "gk_device.hpp"
 
 namespace gk {
@@ -23,10 +23,12 @@ namespace gk {
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// Neighbour-lane values through DPP wavefront shifts: one VALU move per dword instead of an LDS
-// permute and its wait.  lane_below gives lane i the value of lane i-1 (wave_shr:1), lane_above
-// that of lane i+1 (wave_shl:1); the end lane with no source (0, resp. 63) keeps its own value,
-// as __shfl_up/__shfl_down return it.  Every lane of the wave must be active.
+// Neighbour-lane values: lane_below gives lane i the value of lane i-1, lane_above that of lane
+// i+1; the end lane with no source (0, resp. 63) keeps its own value, as __shfl_up/__shfl_down
+// return it.  Doubles move through DPP wavefront shifts (wave_shr:1 / wave_shl:1), two VALU moves
+// instead of two LDS permutes and their wait: the FP64 stage kernels run 2-3% faster.  Floats keep
+// the LDS permute: with DPP moves the FP32 stage kernels ran 7-10% slower.  Every lane of the
+// wave must be active.
 template <int CTRL>
 __device__ __forceinline__ double dpp_shift(double v) {
     const int lo = __double2loint(v), hi = __double2hiint(v);
@@ -38,8 +40,10 @@ __device__ __forceinline__ float dpp_shift(float v) {
     const int b = __float_as_int(v);
     return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xf, 0xf, false));
 }
-template <class R> __device__ __forceinline__ R lane_below(R v) { return dpp_shift<0x138>(v); }
-template <class R> __device__ __forceinline__ R lane_above(R v) { return dpp_shift<0x130>(v); }
+__device__ __forceinline__ double lane_below(double v) { return dpp_shift<0x138>(v); }
+__device__ __forceinline__ double lane_above(double v) { return dpp_shift<0x130>(v); }
+__device__ __forceinline__ float lane_below(float v) { return __shfl_up(v, 1); }
+__device__ __forceinline__ float lane_above(float v) { return __shfl_down(v, 1); }
 
 // Max over the 64 lanes of a wave (all lanes get the result).
 __device__ __forceinline__ double wave_max(double v) {
@@ -709,14 +713,14 @@ __global__ void selftest_kernel(long long n, unsigned long long seed, int emax, 
     unsigned long long local = 0;
     const int emax32 = emax < 60 ? emax : 60;
     {
-        // the stage kernel's DPP lane shifts against the cross-lane shuffles (every lane active)
+        // the DPP lane shifts against the cross-lane shuffles (every lane active)
         const unsigned long long h = mix64(~seed ^ (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x));
         const double v = draw(h, emax, true);
         const float f = drawf((unsigned)h, (unsigned)(h >> 32), emax32);
         local += dbits(lane_below(v)) != dbits(__shfl_up(v, 1));
         local += dbits(lane_above(v)) != dbits(__shfl_down(v, 1));
-        local += __float_as_uint(lane_below(f)) != __float_as_uint(__shfl_up(f, 1));
-        local += __float_as_uint(lane_above(f)) != __float_as_uint(__shfl_down(f, 1));
+        local += __float_as_uint(dpp_shift<0x138>(f)) != __float_as_uint(__shfl_up(f, 1));
+        local += __float_as_uint(dpp_shift<0x130>(f)) != __float_as_uint(__shfl_down(f, 1));
     }
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
         const double a = draw(mix64(seed ^ (3ull * k)), emax, true);

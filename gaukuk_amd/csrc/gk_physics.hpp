@@ -265,6 +265,8 @@ __host__ __device__ __forceinline__ void bc_transform_lanes(R u[5], int comp, in
 // like the reference's registration fallback (src/boundary/boundary.cpp:27-29).
 __host__ __device__ __forceinline__ int bc_remap(int k, int n, int lo, int hi, int& kind) {
     kind = 0;
+    // active index: one unsigned compare, the face-type logic stays off the common path
+    if (__builtin_expect((unsigned)k < (unsigned)n, 1)) return k;
     if (k < 0) {
         if (lo == 1) return k + n;
         if (lo == 2) { kind = 1; return -1 - k; }
