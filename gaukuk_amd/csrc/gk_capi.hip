@@ -32,6 +32,11 @@ struct StageSpec {
 
 }  // namespace
 
+// How one stage launch divides the grid: segments of seg_rows rows, strips x segs (x planes) waves.
+struct WavePlan {
+    int seg_rows = 0, segs = 0, nwaves = 0, nblocks = 0;
+};
+
 struct gk_ctx {
     gk_grid grid;
     gk_phys phys;
@@ -50,6 +55,11 @@ struct gk_ctx {
     DevClock* hclock = nullptr;  // pinned mirror
     hipStream_t stream = nullptr;
     int seg_rows = 0, strips = 0, segs = 0, nwaves = 0, nblocks = 0;
+    bool tile_fixed = false;   // gk_set_tile chose the segment height for every stage kernel
+    int cus = 0;               // compute units of the device
+    // per stage kernel (keyed by its entry point): segments sized to one resident round at that
+    // kernel's occupancy
+    std::vector<std::pair<const void*, WavePlan>> plans;
     bool uploaded = false;
     bool gravity = false;
     double dxRec = 0, dyRec = 0, drmin = 0;
@@ -159,21 +169,56 @@ static void launch_immersed(gk_ctx* c, const StageSpec& s) {
     else launch_immersed_t<double>(c, c->buf[s.in], c->buf[s.out], pingpong, 0);
 }
 
+// Segments for a target number of waves (seg_rows <= 0), or of the given height.  Segments of at
+// least 8 rows; with a target, at most `target` waves, so that they are resident at once, unless
+// that leaves more than a fifth of the target idle (then the next segment count up: a 3D grid of
+// 256 planes x 5 strips runs 2560 waves at a target of 2048 25% faster than 1280).
+static WavePlan make_plan(const gk_ctx* c, int target, int seg_rows) {
+    WavePlan p;
+    if (seg_rows <= 0) {
+        const int per_seg = c->strips * c->nz;
+        int segs = std::max(1, target / per_seg);
+        if (5LL * segs * per_seg < 4LL * target) segs = (target + per_seg - 1) / per_seg;
+        seg_rows = std::max((c->ny + segs - 1) / segs, 8);
+    }
+    p.seg_rows = std::min(seg_rows, c->ny);
+    p.segs = (c->ny + p.seg_rows - 1) / p.seg_rows;
+    p.nwaves = c->strips * p.segs * c->nz;
+    const int nb = (p.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
+    p.nblocks = (nb + 7) / 8 * 8;  // a multiple of 8 keeps the XCD remap a bijection
+    return p;
+}
+
+// The default plan (reported by gk_get_tile, used where no kernel-specific plan applies): one
+// resident round of 3 waves per SIMD on 256 CUs x 4 SIMDs (2 for the 3D variant).
 static void plan_waves(gk_ctx* c, int seg_rows) {
     c->strips = (c->nx + kStrip - 1) / kStrip;
-    if (seg_rows <= 0) {
-        // one resident round: 3 waves per SIMD on 256 CUs x 4 SIMDs (2 for the 3D variant)
-        const int target = c->nz > 1 ? 2048 : 3072;
-        int segs = (target + c->strips * c->nz - 1) / (c->strips * c->nz);
-        seg_rows = (c->ny + segs - 1) / segs;
-        if (seg_rows < 8) seg_rows = 8;
+    const WavePlan p = make_plan(c, c->nz > 1 ? 2048 : 3072, seg_rows);
+    c->seg_rows = p.seg_rows;
+    c->segs = p.segs;
+    c->nwaves = p.nwaves;
+    c->nblocks = p.nblocks;
+    c->tile_fixed = seg_rows > 0;
+    c->plans.clear();
+}
+
+// The plan of one stage kernel: as many waves as the device holds at once at that kernel's
+// occupancy (its register count decides it: 3 waves per SIMD for the FP64 2D kernels, 4-6 for
+// FP32), so that every wave starts in the first round; the default plan if the runtime cannot
+// say, or the fixed height of gk_set_tile.
+static const WavePlan& stage_plan(gk_ctx* c, const void* kernel) {
+    for (const auto& e : c->plans)
+        if (e.first == kernel) return e.second;
+    WavePlan p;
+    int blocks = 0;
+    if (!c->tile_fixed && c->cus > 0 &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kBlock, 0) == hipSuccess && blocks > 0) {
+        p = make_plan(c, blocks * (kBlock / kWave) * c->cus, 0);
+    } else {
+        p.seg_rows = c->seg_rows; p.segs = c->segs; p.nwaves = c->nwaves; p.nblocks = c->nblocks;
     }
-    if (seg_rows > c->ny) seg_rows = c->ny;
-    c->seg_rows = seg_rows;
-    c->segs = (c->ny + seg_rows - 1) / seg_rows;
-    c->nwaves = c->strips * c->segs * c->nz;
-    int nb = (c->nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
-    c->nblocks = (nb + 7) / 8 * 8;  // a multiple of 8 keeps the XCD remap a bijection
+    c->plans.emplace_back(kernel, p);
+    return c->plans.back().second;
 }
 
 static void build_integrator(gk_ctx* c) {
@@ -236,14 +281,14 @@ using PlmLimiter = VanLeer;
 
 template <class R, class L, bool THREE, int VAR>
 static void launch_stage_f(gk_ctx* c, const StageArgs<R>& a, bool last) {
-    dim3 grid(c->nblocks), block(kBlock);
-    if (c->gravity) {
-        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, true, VAR>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, true, false, VAR>), grid, block, 0, c->stream, a);
-    } else {
-        if (last) hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, true, VAR>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((stage_kernel<R, L, THREE, false, false, VAR>), grid, block, 0, c->stream, a);
-    }
+    void (*k)(StageArgs<R>) =
+        c->gravity ? (last ? stage_kernel<R, L, THREE, true, true, VAR> : stage_kernel<R, L, THREE, true, false, VAR>)
+                   : (last ? stage_kernel<R, L, THREE, false, true, VAR> : stage_kernel<R, L, THREE, false, false, VAR>);
+    const WavePlan& p = stage_plan(c, reinterpret_cast<const void*>(k));
+    StageArgs<R> b = a;
+    b.seg_rows = p.seg_rows;
+    b.nwaves = p.nwaves;
+    hipLaunchKernelGGL(k, dim3(p.nblocks), dim3(kBlock), 0, c->stream, b);
 }
 
 template <class R, class L, bool THREE>
@@ -502,6 +547,10 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
         return s;
     };
     if (hipStreamCreate(&c->stream) != hipSuccess) return bail(GK_EHIP, "hipStreamCreate failed");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        c->cus = 0;  // stage kernels then use the default plan
     const size_t bytes = (size_t)c->plane * 5 * c->esz;
     const int nbuf = phys->integrator == 3 ? 3 : 2;
     for (int k = 0; k < nbuf; k++)
