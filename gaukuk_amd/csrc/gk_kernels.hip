@@ -326,10 +326,10 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             if (THREE) res[n] = a.c1 * b2[n] + a.c2 * ut[n] - a.c3 * dF;
             else res[n] = a.c1 * b2[n] - a.c2 * dF;
         }
-        a.eos.floors(res);
+        const R rinv = a.eos.floors(res);
         if (GRAV && LAST) kick(res, 1, cown, r - 2);
         if (LAST) {
-            const R ws = a.eos.wave_speed(res);
+            const R ws = a.eos.wave_speed(res, rinv);
             cacc = owner ? smax(cacc, ws) : cacc;
         }
         if (owner) {

@@ -838,8 +838,8 @@ __global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArg
             if (THREE) res[n] = a.c1 * bse[n] + a.c2 * ut - a.c3 * dF;
             else res[n] = a.c1 * ut - a.c2 * dF;
         }
-        a.eos.floors(res);
-        if (LAST) cacc = a.eos.wave_speed(res);
+        const float rinv = a.eos.floors(res);
+        if (LAST) cacc = a.eos.wave_speed(res, rinv);
 #pragma unroll
         for (int n = 0; n < 5; n++) out[o + n * a.plane] = res[n];
     }

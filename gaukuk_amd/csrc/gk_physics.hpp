@@ -122,8 +122,10 @@ struct Adiabatic {
 
     // One cell's max_d |v_d| + c_s with CalCmax's own kinetic-energy form: adiabatic.cpp:115-160.
     __host__ __device__ __forceinline__ R wave_speed(const R u[5]) const {
-        R d = smax(u[0], R(GK_DENSITY_FLOOR));
-        R inv = qdiv(R(1.0), d);
+        return wave_speed(u, qdiv(R(1.0), smax(u[0], R(GK_DENSITY_FLOOR))));
+    }
+    // The same with inv = 1 / max(u[0], floor) already formed (floors() returns it).
+    __host__ __device__ __forceinline__ R wave_speed(const R u[5], R inv) const {
         R v1 = u[1] * inv, v2 = u[2] * inv, v3 = u[3] * inv;
         R ek = R(0.5) * (v1 * u[1] + v2 * u[2] + v3 * u[3]);
         R p = gm1 * (u[4] - ek);
@@ -138,7 +140,9 @@ struct Adiabatic {
     // Post-update floors of UpdateCons: src/evolution/update_cons.cpp:55-67, written without
     // branches (both outcomes are computed and selected; the quotient 1/r[0] of the low-density
     // case is discarded).  pm_gm1 = pmin/gm1 is the same quotient the reference forms in place.
-    __host__ __device__ __forceinline__ void floors(R r[5]) const {
+    // Returns 1 / max(r[0], floor) of the floored state, the quotient wave_speed forms (the
+    // source kicks after the floors leave the density unchanged).
+    __host__ __device__ __forceinline__ R floors(R r[5]) const {
         const R dmin = R(GK_DENSITY_FLOOR), pmin = R(GK_PRESSURE_FLOOR);
         const R pm_gm1 = pmin / gm1;
         const bool low_d = r[0] < dmin;
@@ -151,6 +155,7 @@ struct Adiabatic {
         r[2] = low_d ? R(0.0) : r[2];
         r[3] = low_d ? R(0.0) : r[3];
         r[0] = low_d ? dmin : r[0];
+        return low_d ? R(1.0) / dmin : inv;
     }
 };
 
