@@ -127,8 +127,11 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     // column of the combine operands: every lane loads them (no branch), so halo lanes clamp
     // into [0, nx-1]; only owner lanes store
     const int cown = min(max(c, 0), a.nx - 1);
-    // only the first and last strips hold ghost columns: the x transform is a uniform branch
+    // only the first and last strips hold ghost columns, and only reflective and outflow faces
+    // transform them: the x transform is a uniform branch (periodic, copy and fixed faces skip it,
+    // so their edge strips run no longer than the others)
     const bool xedge = strip == 0 || strip == a.strips - 1;
+    const bool xtrans = xedge && ((a.bc[0] == 2) | (a.bc[0] == 3) | (a.bc[1] == 2) | (a.bc[1] == 3));
 
     // Addresses are a wave-uniform row pointer (scalar registers) plus a per-lane 32-bit byte
     // offset, which the compiler turns into scalar-base loads and stores.
@@ -186,7 +189,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             const int rs = bc_remap(r, a.ny, a.bc[2], a.bc[3], kyy);
             if (!FIXED || (kyy != 4 && !(xfixed && kx == 4) && !inner_at(r))) kick(u, 0, cs, rs);
         }
-        if (xedge) bc_transform_lanes(u, 1, kx);
+        if (xtrans) bc_transform_lanes(u, 1, kx);
         if (r < 0 || r >= a.ny) {
             int ky;
             bc_remap(r, a.ny, a.bc[2], a.bc[3], ky);
