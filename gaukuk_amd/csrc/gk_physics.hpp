@@ -234,7 +234,7 @@ __host__ __device__ __forceinline__ void bc_transform(R u[5], int comp, int kind
         const R m = u[comp];
         const R mc = (kind == 2) ? smin(m, R(0.0)) : smax(m, R(0.0));
         const R d = smax(u[0], R(GK_DENSITY_FLOOR));
-        const R inv = R(1.0) / d;
+        const R inv = qdiv(R(1.0), d);
         const R keOld = R(0.5) * inv * m * m;
         const R keNew = R(0.5) * inv * mc * mc;
         R eint = u[4] - keOld;
@@ -252,7 +252,7 @@ __host__ __device__ __forceinline__ void bc_transform_lanes(R u[5], int comp, in
     const R m = u[comp];
     const R mc = (kind == 2) ? smin(m, R(0.0)) : smax(m, R(0.0));
     const R d = smax(u[0], R(GK_DENSITY_FLOOR));
-    const R inv = R(1.0) / d;
+    const R inv = qdiv(R(1.0), d);
     const R keOld = R(0.5) * inv * m * m;
     const R keNew = R(0.5) * inv * mc * mc;
     R eint = u[4] - keOld;

@@ -129,7 +129,10 @@ gk_status gk_cons_sums(gk_ctx* ctx, double out[5]);
 /* GK_ENUMERIC if any active conserved value is NaN or infinite. */
 gk_status gk_check_finite(gk_ctx* ctx);
 
-/* Tuning: rows per wave segment of the stage kernel (0 = automatic). */
+/* Tuning: rows per wave segment of every stage kernel (0 = automatic: each stage kernel gets the
+ * segment height that fills the device once at that kernel's occupancy).  Results do not depend
+ * on the choice.  gk_get_tile reports the fixed height, or in automatic mode the plan for three
+ * waves per SIMD (the FP64 2D kernels' occupancy; two for 3D). */
 gk_status gk_set_tile(gk_ctx* ctx, int seg_rows);
 gk_status gk_get_tile(gk_ctx* ctx, int* seg_rows, int* strips, int* waves);
 
