@@ -84,6 +84,17 @@ def test_tile_shape_invariance(hip, seg_rows):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("seg_rows", [5, 13])
+def test_tile_shape_invariance_fp32_gravity(hip, seg_rows):
+    """FP32 with gravity: the automatic plans give each RK2 stage kernel its own segment height
+    (their occupancies differ); a fixed height for both must give the same bits."""
+    prob = P.rayleigh_taylor(1024, 3072, stepmax=10)   # automatic heights 11 and 14 rows
+    sa = hip.Solver(prob, precision=32).run_to_end()
+    sb = hip.Solver(prob, precision=32, seg_rows=seg_rows).run_to_end()
+    assert sa.step == sb.step == 10
+    assert np.array_equal(sa.download(), sb.download())
+
+
 def test_run_to_run_bitwise(hip):
     prob = P.kelvin_helmholtz(200, seed=9, integrator=3, stepmax=50)
     a = hip.Solver(prob).run_to_end().download()
