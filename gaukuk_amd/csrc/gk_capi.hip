@@ -317,15 +317,26 @@ static void launch_stage_any(gk_ctx* c, const StageSpec& s) {
     else launch_stage<double>(c, s);
 }
 
-static void enqueue_step(gk_ctx* c) {
-    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
+static void enqueue_stages(gk_ctx* c) {
     for (size_t q = 0; q < c->stages.size(); q++) {
         if (c->recording) record_launch(c, q);
         launch_immersed(c, c->stages[q]);
         launch_stage_any(c, c->stages[q]);
     }
-    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap,
-                       c->phys.integrator == 1 ? 1 : 0);
+}
+
+// n steps back to back: the loop test and dt of the first step, each later step's joined to the
+// previous step's tail in one launch, and the last step's tail
+static void enqueue_steps(gk_ctx* c, int n) {
+    if (n <= 0) return;
+    const int toggle = c->phys.integrator == 1 ? 1 : 0;
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
+    for (int k = 0; k < n; k++) {
+        if (k > 0)
+            hipLaunchKernelGGL(step_seam_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap, toggle);
+        enqueue_stages(c);
+    }
+    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap, toggle);
 }
 
 static gk_status read_clock(gk_ctx* c) {
@@ -694,7 +705,7 @@ gk_status gk_advance(gk_ctx* c, double dtoutput, double* t, int* step) {
     int chunk = 4;
     int last_step = -1;
     for (;;) {
-        for (int k = 0; k < chunk; k++) enqueue_step(c);
+        enqueue_steps(c, chunk);
         GK_TRY_HIP(c, hipGetLastError());
         gk_status s = read_clock(c);
         if (s != GK_OK) return s;
@@ -962,7 +973,7 @@ gk_status gk_run_steps(gk_ctx* c, int nsteps) {
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
     if (gk_status fs = check_fixed(c)) return fs;
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
-    for (int k = 0; k < nsteps; k++) enqueue_step(c);
+    enqueue_steps(c, nsteps);
     GK_TRY_HIP(c, hipGetLastError());
     return GK_OK;
 }
@@ -991,8 +1002,11 @@ gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kerne
         ev.push_back(e);
     };
     rec();
+    const int toggle = c->phys.integrator == 1 ? 1 : 0;
     for (int k = 0; k < nsteps; k++) {
-        hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
+        // the same launches as enqueue_steps
+        if (k == 0) hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
+        else hipLaunchKernelGGL(step_seam_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap, toggle);
         rec(); kind.push_back(3);
         for (size_t q = 0; q < c->stages.size(); q++) {
             if (c->nshell) {
@@ -1002,8 +1016,9 @@ gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kerne
             launch_stage_any(c, c->stages[q]);
             rec(); kind.push_back((int)q);
         }
-        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap,
-                           c->phys.integrator == 1 ? 1 : 0);
+    }
+    if (nsteps > 0) {
+        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap, toggle);
         rec(); kind.push_back(3);
     }
     GK_TRY_HIP(c, hipGetLastError());
