@@ -127,10 +127,11 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     // column of the combine operands: every lane loads them (no branch), so halo lanes clamp
     // into [0, nx-1]; only owner lanes store
     const int cown = min(max(c, 0), a.nx - 1);
-    // only the first and last strips hold ghost columns, and only reflective and outflow faces
-    // transform them: the x transform is a uniform branch (periodic, copy and fixed faces skip it,
-    // so their edge strips run no longer than the others)
-    const bool xedge = strip == 0 || strip == a.strips - 1;
+    // ghost columns sit in the first strip and in every strip whose last lane reaches column nx
+    // (the last strip, and also the one before it when nx = 1 mod 60); only reflective and outflow
+    // faces transform them: the x transform is a uniform branch (periodic, copy and fixed faces
+    // skip it, so their edge strips run no longer than the others)
+    const bool xedge = strip == 0 || strip * kStrip + (kWave - 1 - kHalo) >= a.nx;
     const bool xtrans = xedge && ((a.bc[0] == 2) | (a.bc[0] == 3) | (a.bc[1] == 2) | (a.bc[1] == 3));
 
     // Addresses are a wave-uniform row pointer (scalar registers) plus a per-lane 32-bit byte

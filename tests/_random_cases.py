@@ -10,7 +10,7 @@ import numpy as np
 
 from gaukuk_amd import problems as P
 
-CASES = 200
+CASES = 1000
 
 
 def random_problem(seed):

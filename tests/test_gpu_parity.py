@@ -202,6 +202,32 @@ def test_fixed_faces_vs_oracle(hip, precision, bc, g):
         assert np.array_equal(out[:, -n0:, n0:-n0], snap[:, -n0:, n0:-n0].astype(out.dtype))
 
 
+@pytest.mark.parametrize("nx,bc", [(121, (3, 2, 1, 1)), (181, (2, 3, 2, 2)), (241, (0, 2, 1, 1)),
+                                   (121, (1, 4, 1, 1))])
+def test_ghost_column_in_strip_before_last(hip, nx, bc):
+    """nx = 1 mod 60: the last lane of the strip before the last holds ghost column nx, which a
+    reflective, outflow or fixed right face must fill like the last strip's ghost columns (the
+    shear flow puts momentum on that face)."""
+    from oracle import gko
+    prob = P.kelvin_helmholtz(nx, ny=40, seed=3, integrator=3, stepmax=20).with_(bc=bc)
+    print(f"ghost column nx={nx} bc={bc}", flush=True)
+    ref = gko.OracleSim(prob)
+    s = hip.Solver(prob, upload=False)
+    if 4 in bc:
+        snap = _fixed_snapshot(prob, 5)
+        ref.enroll_fixed(snap)
+        full = np.zeros(s._ref_shape())
+        full[:, prob.ng] = snap
+        s.set_fixed_boundary(full)
+    ref.run_to_end()
+    s.upload(prob.cons)
+    s.run_to_end()
+    assert s.step == ref.step == 20
+    err = rel_l1(s.download(), ref.cons())
+    print("rel L1", err)
+    assert np.all(err <= TOL64), err
+
+
 def test_fixed_faces_need_data(hip):
     prob = P.blast(64, 64, bc=(4, 4, 1, 1), stepmax=2)
     s = hip.Solver(prob)

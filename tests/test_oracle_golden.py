@@ -47,7 +47,7 @@ def test_eos_vectors():
 
 
 def test_random_configurations_match_reference():
-    """200 seeded random configurations (1D/2D/3D, every face type, integrator, order, gravity,
+    """1000 seeded random configurations (1D/2D/3D, every face type, integrator, order, gravity,
     point masses, inner region, immersed shell): the oracle's step count, final time and state
     checksums equal the compiled reference's."""
     from tests._random_cases import CASES, checksums, describe, random_problem
