@@ -42,6 +42,24 @@ def test_driver_golden(tmp_path, name):
     assert np.all(err <= 1e-12), err
 
 
+@pytest.mark.parametrize("seed", range(40))
+def test_driver_random_configs(tmp_path, seed):
+    """The drop-in driver (setup, .in file, output file) on the seeded random configurations
+    (tests/_random_cases.py) against the oracle: step count, final time, relative L1 <= 1e-12."""
+    from oracle import gko
+    from tests._random_cases import describe, random_problem
+    prob = random_problem(seed)
+    print(describe(seed, prob), flush=True)
+    out, total = run_from_file(str(tmp_path), prob, extra_args=("--quiet",))
+    files = sorted(f for f in os.listdir(tmp_path) if f.startswith("cons_"))
+    o = GaukukOutput(str(tmp_path / files[-1]))
+    ref = gko.OracleSim(prob).run_to_end()
+    assert total == ref.step
+    assert abs(o.t - ref.t) <= 1e-12 * ref.t
+    err = rel_l1(o.active(), ref.cons())
+    assert np.all(err <= 1e-12), err
+
+
 def test_driver_sod_setup_matches_reference(tmp_path):
     prob, final, t, steps = load_run("sod256")
     shutil.copy(os.path.join(HOST, "input", "shock_tube.in"), tmp_path / "shock_tube.in")
