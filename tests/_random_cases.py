@@ -78,3 +78,13 @@ def checksums(cons: np.ndarray) -> np.ndarray:
     u = np.asarray(cons, np.float64).reshape(5, -1)
     w = np.random.default_rng(7).uniform(-1.0, 1.0, u.shape[1])
     return np.concatenate([np.abs(u).sum(axis=1), u @ w])
+
+
+def with_outputs(prob, seed):
+    """The same problem run to a random end time in 1-6 outputs, without or with a step limit
+    (the loop test, dt clipped to each output time, several advance calls)."""
+    rng = np.random.default_rng(99000 + seed)
+    tmax = float(rng.uniform(0.005, 0.08))
+    nout = int(rng.integers(1, 7))
+    stepmax = int(rng.choice([-1, -1, int(rng.integers(3, 30))]))
+    return prob.with_(tmax=tmax, dtoutput=tmax / nout, stepmax=stepmax)
