@@ -206,7 +206,7 @@ static void plan_waves(gk_ctx* c, int seg_rows) {
 // occupancy (its register count decides it: 3 waves per SIMD for the FP64 2D kernels, 4-6 for
 // FP32), so that every wave starts in the first round; the default plan if the runtime cannot
 // say, or the fixed height of gk_set_tile.
-static const WavePlan& stage_plan(gk_ctx* c, const void* kernel) {
+static WavePlan stage_plan(gk_ctx* c, const void* kernel) {
     for (const auto& e : c->plans)
         if (e.first == kernel) return e.second;
     WavePlan p;
@@ -218,7 +218,7 @@ static const WavePlan& stage_plan(gk_ctx* c, const void* kernel) {
         p.seg_rows = c->seg_rows; p.segs = c->segs; p.nwaves = c->nwaves; p.nblocks = c->nblocks;
     }
     c->plans.emplace_back(kernel, p);
-    return c->plans.back().second;
+    return p;
 }
 
 static void build_integrator(gk_ctx* c) {
@@ -284,7 +284,7 @@ static void launch_stage_f(gk_ctx* c, const StageArgs<R>& a, bool last) {
     void (*k)(StageArgs<R>) =
         c->gravity ? (last ? stage_kernel<R, L, THREE, true, true, VAR> : stage_kernel<R, L, THREE, true, false, VAR>)
                    : (last ? stage_kernel<R, L, THREE, false, true, VAR> : stage_kernel<R, L, THREE, false, false, VAR>);
-    const WavePlan& p = stage_plan(c, reinterpret_cast<const void*>(k));
+    const WavePlan p = stage_plan(c, reinterpret_cast<const void*>(k));
     StageArgs<R> b = a;
     b.seg_rows = p.seg_rows;
     b.nwaves = p.nwaves;
