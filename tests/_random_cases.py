@@ -3,7 +3,7 @@ golden checksums.
 
 Each case draws grid sizes (1D, 2D and 3D, sizes around the 60-column strip width), face types
 0-3 per face (0-4 in 2D, type 4 with random saved ghost data), the integrator, the reconstruction
-order, constant gravity, point masses, and in 2D the inner fixed region and an immersed shell,
+order, constant gravity, point and binary masses, and in 2D the inner fixed region and an immersed shell,
 plus a smooth random state with a pressure bump; every case runs 8 steps.
 """
 import numpy as np
@@ -44,7 +44,12 @@ def random_problem(seed):
     g = tuple(float(v) for v in (rng.uniform(-0.2, 0.2, 3) if rng.random() < 0.4 else (0, 0, 0)))
     point = ()
     if kind != "1d" and rng.random() < 0.3:
-        point = ((float(rng.uniform(0.05, 0.2)), float(0.5 * nx / L), float(0.5 * ny / L), -0.3, 0, 0, 0, 0.1),)
+        m0 = (float(rng.uniform(0.05, 0.2)), float(rng.uniform(0.3, 0.7) * nx / L), float(rng.uniform(0.3, 0.7) * ny / L),
+              float(rng.uniform(-0.3, 0.3)), 0, 0, 0, float(rng.uniform(0.05, 0.15)))
+        point = (m0,)
+        if rng.random() < 0.5:   # a second mass: binary gravity
+            point = (m0, (float(rng.uniform(0.01, 0.05)), float(rng.uniform(0.1, 0.9) * nx / L),
+                          float(rng.uniform(0.1, 0.9) * ny / L), m0[3], 0, 0, 0, float(rng.uniform(0.02, 0.08))))
     fixed, fix_inner, shell = (), False, ()
     if plane and rng.random() < 0.25:
         fix_inner = True
