@@ -14,8 +14,8 @@
 // consecutive rows; the y-face flux between rows r-2 and r-1 and the x-face fluxes of row r-2
 // are computed once per face, so every cell is converted once and every face solved once
 // (plus the strip halo).  Loads of the next row are issued before the current row's flux math.
-// Waves never exchange data, so there is no LDS allocation and no barrier; x neighbours move
-// between lanes (DPP shifts in FP64, permutes in FP32).
+// Waves never exchange data, so there is no barrier; x neighbours move between lanes (DPP shifts
+// in FP64, permutes in FP32), and only the 3D variant uses LDS, as per-wave private storage.
 #include "gk_device.hpp"
 
 namespace gk {
@@ -71,7 +71,8 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 // `strip` and `seg` are wave-uniform (scalar registers), so row logic is scalar branches and
 // scalar address arithmetic.
 template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR>
-__device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane, int kp = 0) {
+__device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane, int kp = 0,
+                                      R* zring = nullptr) {
     constexpr bool FIXED = VAR == 1;
     constexpr bool THREED = VAR == 2;
     constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
@@ -277,15 +278,29 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             for (int n = 0; n < 5; n++) u[n] = readsX ? a.zstale[o + n * pl] : R(0);
         }
     };
-    // W -+ sigma_y/2 of (plane kz, row jz)
-    auto zstate = [&](int kz, int jz, R lo[5], R hi[5]) {
-        R u0[5], u1[5], u2[5], w0[5], w1[5], w2[5];
-        zload(kz, jz - 1, u0);
-        zload(kz, jz, u1);
-        zload(kz, jz + 1, u2);
-        a.eos.cons_to_prim(u0, w0);
-        a.eos.cons_to_prim(u1, w1);
-        a.eos.cons_to_prim(u2, w2);
+    // Primitive rows of the two neighbour planes (pl 0: kp-1, 1: kp+1), rows j0-1 .. j1, in a
+    // three-row ring in this wave's LDS slice ([2][3][5][64], each lane its own column): every
+    // row is loaded and converted once per segment instead of three times.
+    auto zslot = [&](int pl, int row, int n) -> R& {
+        return zring[((pl * 3 + (row - j0 + 1) % 3) * 5 + n) * kWave + lane];
+    };
+    auto zrow = [&](int pl, int row) {
+        R u[5], w[5];
+        zload(pl ? kp + 1 : kp - 1, row, u);
+        a.eos.cons_to_prim(u, w);
+#pragma unroll
+        for (int n = 0; n < 5; n++) zslot(pl, row, n) = w[n];
+    };
+    // W -+ sigma_y/2 of (neighbour plane pl, row jz): rows jz-1, jz from the ring, row jz+1 added
+    auto zstate = [&](int pl, int jz, R lo[5], R hi[5]) {
+        zrow(pl, jz + 1);
+        R w0[5], w1[5], w2[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            w0[n] = zslot(pl, jz - 1, n);
+            w1[n] = zslot(pl, jz, n);
+            w2[n] = zslot(pl, jz + 1, n);
+        }
         yslope(w0, w1, w2, lo, hi);
     };
     // z-face flux with the direction-VLZ rotation (hllc.cpp:25-27): (den, vz, vx, vy, p)
@@ -299,9 +314,9 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     // the z-flux difference H(k+1/2) - H(k-1/2) of row rr, from this plane's states (lo, hi)
     auto zfluxes = [&](int rr, const R lo_here[5], const R hi_here[5], R Hz[5]) {
         R lo[5], hi[5], Hd[5], Hu[5];
-        zstate(kp - 1, rr, lo, hi);
+        zstate(0, rr, lo, hi);
         zflux(hi, lo_here, Hd);
-        zstate(kp + 1, rr, lo, hi);
+        zstate(1, rr, lo, hi);
         zflux(hi_here, lo, Hu);
 #pragma unroll
         for (int n = 0; n < 5; n++) Hz[n] = Hu[n] - Hd[n];
@@ -387,6 +402,11 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             for (int n = 0; n < 5; n++) wRz[n] = wR[n];
         }
         yflux(wLy, wR, Gp);            // face j0
+        if (THREED) {
+            // the neighbour planes' rows below the first update row
+#pragma unroll
+            for (int pl = 0; pl < 2; pl++) { zrow(pl, j0 - 1); zrow(pl, j0); }
+        }
 #pragma unroll
         for (int n = 0; n < 5; n++) wLy[n] = wLn[n];
     }
@@ -431,11 +451,13 @@ void stage_kernel(const StageArgs<R> a) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
     if (w >= a.nwaves) return;
-    if (VAR == 2) {
-        // 3D: strip fastest, then segment, then plane (waves of neighbouring planes share an XCD)
+    if constexpr (VAR == 2) {
+        // 3D: strip fastest, then segment, then plane (waves of neighbouring planes share an XCD);
+        // each wave keeps its neighbour-plane row ring in its own LDS slice
+        __shared__ R zring[kBlock / kWave][2 * 3 * 5 * kWave];
         const int per_plane = a.nwaves / a.nz;
         const int kp = w / per_plane, r = w - kp * per_plane;
-        march<R, Lim, THREE, GRAV, LAST, VAR>(a, r % a.strips, r / a.strips, lane, kp);
+        march<R, Lim, THREE, GRAV, LAST, VAR>(a, r % a.strips, r / a.strips, lane, kp, zring[wid]);
     } else {
         march<R, Lim, THREE, GRAV, LAST, VAR>(a, w % a.strips, w / a.strips, lane);
     }
