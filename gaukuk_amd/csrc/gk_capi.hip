@@ -170,7 +170,7 @@ static void launch_immersed(gk_ctx* c, const StageSpec& s) {
 }
 
 // Segments for a target number of waves (seg_rows <= 0), or of the given height.  Segments of at
-// least 8 rows; with a target, at most `target` waves, so that they are resident at once, unless
+// least 8 rows (2 on small grids, below); with a target, at most `target` waves, so that they are resident at once, unless
 // that leaves more than a fifth of the target idle: then about five rounds of shorter segments,
 // whose waves refill the slots as others finish (a 3D grid of 256 planes x 5 strips at a target
 // of 2048: one round of 1280 waves 3.3, 2560 waves 3.3, 10240 waves of 32 rows 4.2 G zone-updates/s).
@@ -181,6 +181,11 @@ static WavePlan make_plan(const gk_ctx* c, int target, int seg_rows) {
         long long segs = std::max(1, target / per_seg);
         if (5LL * segs * per_seg < 4LL * target) segs = (5LL * target + per_seg - 1) / per_seg;
         seg_rows = std::max((int)((c->ny + segs - 1) / segs), 8);
+        // a grid whose 8-row segments fill less than an eighth of the slots: 2-row segments (KH
+        // 256^2 FP64: 160 waves 55.9, 640 waves 32.9 us per RK3 step; at 512^2, 576 waves of 8
+        // rows are already faster than 2304 of 2)
+        const long long waves8 = (long long)per_seg * ((c->ny + 7) / 8);
+        if (8 * waves8 < target) seg_rows = 2;
     }
     p.seg_rows = std::min(seg_rows, c->ny);
     p.segs = (c->ny + p.seg_rows - 1) / p.seg_rows;
