@@ -2,11 +2,15 @@
 """Benchmark of the per-timestep hydro update on one MI355X.
 
 Workload: Kelvin-Helmholtz 2048x2048, FP64, PLM + HLLC + RK3, CFL 0.4, periodic, seeded ICs.
-A step is one full RK3 step (3 fused stage kernels + 2 one-thread clock kernels) over the
-whole grid with the state resident in HBM.  Prints one JSON line:
-  value         whole-job zone-updates/s over exactly --steps timed steps
+A step is one full RK3 step (3 fused stage kernels + 1 one-thread clock kernel) over the
+whole grid with the state resident in HBM.  Before the --warmup steps a second solver of the same
+workload steps untimed for --prewarm-s seconds of wall time, so that the GPU's clocks have settled
+when the timed solver starts from its initial state.  Prints one JSON line:
+  value         whole-job zone-updates/s over exactly --steps timed steps (enqueued back to back,
+                one synchronize)
   roofline      the stage kernel with the most total time, its algorithmic bytes per launch
-                divided by its average launch duration from HIP events on the library's stream
+                divided by its average launch duration from HIP events on the library's stream,
+                measured over a second pass of --steps steps right after the timed one
   cpu_baseline  the oracle restatement of the reference's OpenMP CPU path (fast flags) timed on
                 this host's cores over a bounded sample of the same workload
 """
@@ -38,9 +42,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host cores this process may run on: logical CPUs in its affinity mask, the physical
+    cores they belong to, and the CPU model (from /proc/cpuinfo)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    model, cores, cur = "unknown", set(), {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().split("\n") + [""]:
+                if not line.strip():
+                    if "processor" in cur and int(cur["processor"]) in cpus:
+                        cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                        model = cur.get("model name", model)
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return {"logical_cpus": len(cpus), "physical_cores": len(cores) or len(cpus), "model": model}
+
+
 def cpu_baseline(prob, budget_s: float = 12.0, label: str = "KH", precision: int = 64):
-    """Time the oracle (fast build) on this host: a bounded sample of the same workload."""
-    threads = int(os.environ.get("GK_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    """Time the oracle (fast build) on this host: a bounded sample of the same workload.  Threads:
+    one per physical core of this process's affinity mask, at most OMP_NUM_THREADS when the
+    environment sets it (the CPU share of the machine)."""
+    host = host_cpu()
+    threads = int(os.environ.get("GK_CPU_THREADS", "0")) or host["physical_cores"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     from oracle import gko  # test infrastructure: the CPU baseline only
@@ -61,7 +91,49 @@ def cpu_baseline(prob, budget_s: float = 12.0, label: str = "KH", precision: int
     return {"value": zu, "unit": "zone-updates/s", "cores": threads, "kind": "port",
             "sample": f"{label} {prob.nx}x{prob.ny} FP{precision} {rk}, {steps} steps of the oracle restatement of the "
                       f"reference OpenMP path (-O3 -ffast-math -funroll-loops -fopenmp), {el:.1f} s, "
-                      f"setup excluded"}
+                      f"setup excluded",
+            "host": {**host, "omp_threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}}
+
+
+def prewarm(s, seconds: float, chunk: int = 20):
+    """Untimed stepping of solver `s` until `seconds` of wall time have passed.  After an idle
+    period the GPU first runs at its boost clock, then its power management pulls the clock down
+    for some tens of milliseconds before it settles (stage kernels 150-200 us instead of 125-150 us
+    at KH 2048^2); stepping a second solver of the same workload right up to the timed solver's
+    warm-up keeps the timed steps out of that transient without changing their initial state."""
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds:
+        s.run_steps(chunk)
+        s.synchronize()
+        steps += chunk
+    return {"seconds": time.perf_counter() - t0, "steps": steps,
+            "on": "a second solver of the same workload (the timed solver starts from the initial state)"}
+
+
+def ensemble_l1(probs, steps: int):
+    """Relative L1 of the FP32 ensemble against the same ensemble in FP64 after `steps` steps of
+    every problem (each with its own dt, from its own state): per problem the largest of the four
+    varying conserved variables (rho, m_x, m_y, E); max and median over the problems."""
+    from gaukuk_amd import hip
+    res = {}
+    for prec in (32, 64):
+        e = hip.Ensemble(probs, precision=prec)
+        e.run_steps(steps)
+        e.synchronize()
+        res[prec] = e
+    worst, per_var = [], [[] for _ in range(4)]
+    for q in range(len(probs)):
+        a, b = res[32].download(q), res[64].download(q)
+        r = [float(np.abs(a[n] - b[n]).sum() / max(np.abs(b[n]).sum(), 1e-300)) for n in (0, 1, 2, 4)]
+        worst.append(max(r))
+        for k in range(4):
+            per_var[k].append(r[k])
+    for e in res.values():
+        e.close()
+    return {"steps": steps, "max": float(np.max(worst)), "median": float(np.median(worst)),
+            "median_per_variable": {v: float(np.median(x)) for v, x in zip(("rho", "m_x", "m_y", "E"), per_var)},
+            "max_per_variable": {v: float(np.max(x)) for v, x in zip(("rho", "m_x", "m_y", "E"), per_var)}}
 
 
 def bench_ensemble(args):
@@ -94,6 +166,14 @@ def bench_ensemble(args):
     ms_looped = ens.time(kl, hip.LOOPED)
     zu = lambda ms, n: cells * n / (ms * 1e-3)
     per_step_bytes = 100.0 * cells  # FP32 RK2: stage 1 reads + writes 20 B each, stage 2 reads 40 B, writes 20 B
+    ens.close()
+    log("FP32 vs FP64 relative L1 per problem")
+    l1 = ensemble_l1(probs, steps=args.l1_steps)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_ens_stage_kernel_ensemble512_fp32_rk2.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_step")
     out = {
         "metric": METRIC, "value": zu(ms_graph, K), "unit": "zone-updates/s", "n_gpus": 1,
         "steps": K, "warmup": args.warmup, "ms_per_step": ms_graph / K, "higher_is_better": True,
@@ -104,14 +184,17 @@ def bench_ensemble(args):
                    "tile": list(best)},
         "batched_eager_zone_updates_per_s": zu(ms_batched, K),
         "graph_zone_updates_per_s": zu(ms_graph, K),
+        # every problem as its own single-grid run: one launch per stage with that grid's automatic
+        # plan, as gaukuk_amd.hip.Solver would run it
         "looped_single_grid_zone_updates_per_s": zu(ms_looped, kl),
         "speedup_vs_looped": zu(ms_graph, K) / zu(ms_looped, kl),
         "step_hbm_frac": per_step_bytes / (ms_graph / K * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "wall_s_graph": wall_graph,
+        "fp32_vs_fp64_rel_l1": l1,
         # the persistent stage launches dominate the step; their bytes are the whole step's
         "roofline": {"bound": "hbm", "achieved": per_step_bytes / (ms_graph / K * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": per_step_bytes / (ms_graph / K * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "ens_stage_kernel<float, VanLeer, ...> (both RK2 stages, per step)",
+                     "traffic": traffic, "kernel": "ens_stage_kernel<float, VanLeer, ...> (both RK2 stages, per step)",
                      "bytes_per_launch": per_step_bytes / 2, "avg_launch_us": ms_graph / K * 1e3 / 2},
     }
     if not args.no_cpu:
@@ -297,6 +380,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seg-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--l1-steps", type=int, default=20, help="ensemble: steps of the FP32-vs-FP64 comparison")
+    ap.add_argument("--prewarm-s", type=float, default=2.0,
+                    help="untimed stepping (wall seconds) of a second solver before the warm-up steps")
     ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "rt", "learned", "3d"],
                     help="kh: the headline (default); ensemble / rt / learned: the other benchmark configurations")
     args = ap.parse_args()
@@ -320,28 +406,35 @@ def main():
     log("building KH 2048^2 initial conditions")
     prob = P.kelvin_helmholtz(NX, seed=0, integrator=3, stepmax=-1)
     s = hip.Solver(prob, seg_rows=args.seg_rows)
+    warm = hip.Solver(prob, seg_rows=args.seg_rows)
     log(f"library {hip.version()}, tile {s.tile()}")
+    pw = prewarm(warm, args.prewarm_s)
+    log(f"prewarm {pw['steps']} steps in {pw['seconds']:.2f} s (untimed) on a second solver")
     log(f"warmup {args.warmup} steps")
     s.run_steps(args.warmup)
     s.synchronize()
 
+    # the timed region: exactly --steps steps enqueued back to back, one synchronize
     log(f"timed {args.steps} steps")
-    s.synchronize()
     t0 = time.perf_counter()
-    total_ms, kms, cnt = s.profile(args.steps)
+    s.run_steps(args.steps)
     s.synchronize()
     wall = time.perf_counter() - t0
-    assert s.check_finite()
     zones = NX * NY
     value = zones * args.steps / wall
+    # the same number of steps again with a HIP event after every launch (events created before
+    # the first launch), for the per-kernel durations
+    total_ms, kms, cnt = s.profile(args.steps)
+    assert s.check_finite()
+    warm.close()
     kernels = []
     for k, (name, bpc) in enumerate(STAGES):
         avg_ms = kms[k] / max(cnt[k], 1)
         kernels.append({"kernel": name, "launches": cnt[k], "avg_launch_us": avg_ms * 1e3,
                         "bytes_per_launch": bpc * zones,
                         "achieved_GBs": bpc * zones / (avg_ms * 1e-3) / 1e9, "total_ms": kms[k]})
-    log(f"wall {wall * 1e3:.1f} ms, events {total_ms:.1f} ms; " +
-        ", ".join(f"{q['kernel'].split('<')[1][:28]} {q['avg_launch_us']:.1f} us" for q in kernels) +
+    log(f"timed: {wall * 1e3:.2f} ms ({wall * 1e3 / args.steps:.4f} ms/step); event pass {total_ms:.2f} ms; " +
+        ", ".join(f"stage {k + 1} {q['avg_launch_us']:.1f} us" for k, q in enumerate(kernels)) +
         f"; clock kernels {kms[3] / max(cnt[3], 1) * 1e3:.1f} us")
     dom = max(kernels, key=lambda q: q["total_ms"])
 
@@ -372,6 +465,8 @@ def main():
                      "avg_launch_us": dom["avg_launch_us"]},
         "stage_kernels": kernels,
         "step_roofline_frac": (320.0 * zones / (wall / args.steps)) / 1e9 / HBM_PEAK_GBS,
+        "event_pass_ms_per_step": total_ms / args.steps,
+        "prewarm": pw,
     }
     if not args.no_cpu:
         log("CPU baseline (oracle, fast build)")

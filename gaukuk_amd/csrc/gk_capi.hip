@@ -125,8 +125,23 @@ static bool has_fixed_face(const gk_ctx* c) {
 // boundaries beyond the face remaps: fixed faces, the inner fixed region, an immersed shell
 static bool has_extra_boundaries(const gk_ctx* c) { return has_fixed_face(c) || c->inner || c->immersed; }
 
+// What the corrected (learned) stage kernel cannot run: it has no gravity kick, no saved-data
+// boundaries and no z faces, works in FP32 and reconstructs with PLM.  nullptr when it can.
+static const char* net_conflict(const gk_ctx* c) {
+    if (c->prec != 32) return "the corrected stage runs in precision 32";
+    if (c->gravity) return "the corrected stage does not take gravity";
+    if (has_extra_boundaries(c))
+        return "the corrected stage takes face types 0-3 only (no fixed, inner or immersed boundaries)";
+    if (c->nz > 1) return "the corrected stage is 2D only";
+    if (c->phys.rc_order == 1) return "the corrected stage uses PLM (RcOrder 2)";
+    return nullptr;
+}
+
 // Boundary::FixedBDXL..FixedBDYR throw when no data was enrolled (src/boundary/fixed.cpp:18-19)
 static gk_status check_fixed(gk_ctx* c) {
+    // an attached network runs every stage: the configuration must still be one it can run
+    if (c->net)
+        if (const char* m = net_conflict(c)) return fail(c, GK_EINVAL, m);
     if (has_fixed_face(c) && c->fixed_ref.empty())
         return fail(c, GK_ESTATE,
                     "fixed boundary is not initialized: call gk_set_fixed_boundary (EnrollFixedBoundaryData)");
@@ -343,6 +358,21 @@ static void enqueue_steps(gk_ctx* c, int n) {
         enqueue_stages(c);
     }
     hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap, toggle);
+}
+
+// Steps to enqueue before the host looks at the clock again: the steps left to the output time at
+// the next step's dt (from the wave speed the last stage left for it), one more for a dt that
+// shrinks on the way and for the loop test that ends the advance, at most the steps stepmax still
+// allows plus that test, and at most 256 per host round trip.  Steps past the end cost one early
+// return per launch.
+static int advance_chunk(const DevClock& h) {
+    double cm;
+    std::memcpy(&cm, &h.cmax_bits[h.parity], sizeof cm);
+    const double dt = h.cfl * h.drmin / cm;
+    const double rem = h.t_next - h.t;
+    double est = (dt > 0 && std::isfinite(dt)) ? std::ceil(std::max(rem, 0.0) / dt) + 1 : 256;
+    if (h.stepmax > 0) est = std::min(est, (double)(h.stepmax - h.step) + 1);
+    return (int)std::max(1.0, std::min(est, 256.0));
 }
 
 static gk_status read_clock(gk_ctx* c) {
@@ -708,18 +738,17 @@ gk_status gk_advance(gk_ctx* c, double dtoutput, double* t, int* step) {
     if (gk_status fs = check_fixed(c)) return fs;
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, dtoutput);
     GK_TRY_HIP(c, hipGetLastError());
-    int chunk = 4;
+    gk_status s = read_clock(c);
+    if (s != GK_OK) return s;
     int last_step = -1;
-    for (;;) {
-        enqueue_steps(c, chunk);
+    while (c->hclock->active) {
+        enqueue_steps(c, advance_chunk(*c->hclock));
         GK_TRY_HIP(c, hipGetLastError());
-        gk_status s = read_clock(c);
-        if (s != GK_OK) return s;
+        if ((s = read_clock(c)) != GK_OK) return s;
         if (!c->hclock->active) break;
         // an active clock that did not move over a whole chunk would loop forever
         if (c->hclock->step == last_step) return fail(c, GK_ESTATE, "device clock stopped advancing");
         last_step = c->hclock->step;
-        if (chunk < 256) chunk *= 2;
     }
     if (t) *t = c->hclock->t;
     if (step) *step = c->hclock->step;
@@ -770,6 +799,7 @@ gk_status gk_step_log(gk_ctx* c, int* steps, double* t, double* dt, int cap, int
 gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double z, double vx, double vy, double vz,
                                 double rs) {
     if (!c) return GK_EINVAL;
+    if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
     // SourceTerm::EnrollPointGravity (src/source_term/source.cpp:33-67): the first mass gives point
     // gravity, a second turns it into binary gravity, a third is an error
     if (c->npoint >= 2) return fail(c, GK_EINVAL, "Setup failed: too many point gravity.");
@@ -805,6 +835,7 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
 
 gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host, int fix_inner) {
     if (!c || !host) return GK_EINVAL;
+    if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
     if (c->nz > 1) return fail(c, GK_EINVAL, "fixed faces and the inner region are 2D only");
     // Boundary::EnrollFixedBoundaryData: the ghost layers of the given state become the fixed faces'
     // values; the stencil reads the two layers nearest each face
@@ -845,6 +876,7 @@ gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host, int fix_inner) {
 
 gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, double rs) {
     if (!c) return GK_EINVAL;
+    if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
     // Boundary::SetupImmersedShell (src/boundary/immersed.cpp:89-235), nz == 1
     if (c->nx <= 1 || c->ny <= 1) return fail(c, GK_EINVAL, "immersed boundary has to be at least 2D. ");
     if (c->nz > 1) return fail(c, GK_EINVAL, "immersed shells are 2D only");
@@ -997,16 +1029,21 @@ gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kerne
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
     // an event after every launch; the interval ending at a launch's event is its duration.
-    // classes: 0..2 the stage kernels by position in the step, 3 the clock kernels
-    std::vector<hipEvent_t> ev;
+    // classes: 0..2 the stage kernels by position in the step, 3 the clock kernels.  Every event is
+    // created before the first launch, so the timed region holds only launches and records.
+    const size_t per_step = 1 + c->stages.size() * (c->nshell ? 2 : 1);
+    std::vector<hipEvent_t> ev((size_t)nsteps * per_step + 2, nullptr);
+    for (hipEvent_t& e : ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            for (hipEvent_t& d : ev)
+                if (d) (void)hipEventDestroy(d);
+            return fail(c, GK_EHIP, "hipEventCreate failed");
+        }
+    }
     std::vector<int> kind;
-    ev.reserve((size_t)nsteps * 6 + 1);
-    auto rec = [&]() {
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        (void)hipEventRecord(e, c->stream);
-        ev.push_back(e);
-    };
+    kind.reserve(ev.size());
+    size_t nrec = 0;
+    auto rec = [&]() { (void)hipEventRecord(ev[nrec++], c->stream); };
     rec();
     const int toggle = c->phys.integrator == 1 ? 1 : 0;
     for (int k = 0; k < nsteps; k++) {
@@ -1031,14 +1068,14 @@ gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kerne
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
     double kms[4] = {0, 0, 0, 0};
     int cnt[4] = {0, 0, 0, 0};
-    for (size_t q = 0; q + 1 < ev.size(); q++) {
+    for (size_t q = 0; q + 1 < nrec; q++) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
         kms[kind[q]] += ms;
         cnt[kind[q]] += 1;
     }
     float tot = 0;
-    (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
+    if (nrec > 1) (void)hipEventElapsedTime(&tot, ev[0], ev[nrec - 1]);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (total_ms) *total_ms = tot;
     for (int k = 0; k < 4; k++) {

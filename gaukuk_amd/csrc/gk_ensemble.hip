@@ -65,15 +65,9 @@ static void ens_plan(gk_ens* e) {
     }
     e->total_items = base;
     e->blocks = kNumCUs * e->waves_per_cu / (kBlock / kWave);
-    for (int p = 0; p < e->P; p++) {
-        gk_ctx& v = e->views[p];
-        v.seg_rows = e->hprobs[p].seg_rows;
-        v.strips = e->hprobs[p].strips;
-        v.segs = (v.ny + v.seg_rows - 1) / v.seg_rows;
-        v.nwaves = v.strips * v.segs;
-        const int nb = (v.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
-        v.nblocks = (nb + 7) / 8 * 8;
-    }
+    // the looped baseline runs each problem as a single grid would run alone: every stage kernel
+    // with the automatic plan for that grid's size at that kernel's occupancy (stage_plan)
+    for (int p = 0; p < e->P; p++) plan_waves(&e->views[p], 0);
 }
 
 static gk_status ens_upload_table(gk_ens* e) {
@@ -179,6 +173,10 @@ gk_status gk_ens_create(int count, const gk_grid* grids, const gk_phys* phys, gk
     e->hprobs.resize(count);
     e->views.resize(count);
     const int nbuf = c0.integrator == 3 ? 3 : 2;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 0;  // looped launches then use the default plan
     size_t off = 0;
     for (int p = 0; p < count; p++) {
         EnsProblem& d = e->hprobs[p];
@@ -199,7 +197,8 @@ gk_status gk_ens_create(int count, const gk_grid* grids, const gk_phys* phys, gk
         // a single-grid view of the problem for the looped mode (shares the ensemble's memory)
         gk_ctx& v = e->views[p];
         v.grid = grids[p]; v.phys = phys[p]; v.prec = e->prec; v.esz = e->esz;
-        v.nx = d.nx; v.ny = d.ny; v.pitch = d.pitch; v.plane = d.plane;
+        v.nx = d.nx; v.ny = d.ny; v.pitch = d.pitch; v.plane = d.plane; v.plane2 = d.plane;
+        v.cus = cus;
         for (int q = 0; q < 4; q++) v.bc[q] = d.bc[q];
         v.gravity = e->gravity;
         v.dxRec = d.dxRec; v.dyRec = d.dyRec;

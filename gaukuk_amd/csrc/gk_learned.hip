@@ -1386,12 +1386,7 @@ gk_status gk_net_collect(gk_net* n, gk_ctx* fine, gk_ctx* coarse, int steps, dou
 gk_status gk_set_net(gk_ctx* c, gk_net* n) {
     if (!c) return GK_EINVAL;
     if (!n) { c->net = nullptr; return GK_OK; }
-    if (c->prec != 32) return fail(c, GK_EINVAL, "the corrected stage runs in precision 32");
-    if (c->gravity) return fail(c, GK_EINVAL, "the corrected stage does not take gravity");
-    if (has_extra_boundaries(c))
-        return fail(c, GK_EINVAL, "the corrected stage takes face types 0-3 only (no fixed, inner or immersed boundaries)");
-    if (c->nz > 1) return fail(c, GK_EINVAL, "the corrected stage is 2D only");
-    if (c->phys.rc_order == 1) return fail(c, GK_EINVAL, "the corrected stage uses PLM (RcOrder 2)");
+    if (const char* m = net_conflict(c)) return fail(c, GK_EINVAL, m);
     if (c->nx < kLT || c->ny < kLT) return fail(c, GK_EINVAL, "the corrected stage needs NX, NY >= 16");
     if (!c->lstats) {
         const size_t bytes = sizeof(unsigned long long) * 3 * (size_t)learned_tiles(c);

@@ -300,6 +300,10 @@ class Solver:
         self._check(self.L.gk_get_time(self.h, C.byref(t), C.byref(dt), C.byref(s), C.byref(ic)))
         return t.value, dt.value, s.value, bool(ic.value)
 
+    def set_time(self, t: float, step: int):
+        """Set the simulation time and step count (gk_set_time)."""
+        self._check(self.L.gk_set_time(self.h, float(t), int(step)))
+
     @property
     def t(self):
         return self.time()[0]

@@ -38,14 +38,16 @@ typedef enum {
 } gk_status;
 
 /* Grid: the keys NX, NY, NZ, NGhost, xmin..zmax of the .in file (src/grid.hpp:18-41,
- * src/sim.cpp:16-42).  nz must be 1 (2D/1D path); nghost must be 2 (PLM stencil). */
+ * src/sim.cpp:16-42).  nz = 1 runs the 1D/2D path, nz > 1 the 3D path (face types 0-3, constant
+ * and point gravity; ny >= 2); nghost must be at least 2 (PLM stencil). */
 typedef struct {
     int nx, ny, nz, nghost;
     double xmin, xmax, ymin, ymax, zmin, zmax;
 } gk_grid;
 
 /* Physics and integrator: gamma (src/eos/adiabatic.cpp:12-16), CFL, integrator 1/2/3 and
- * RcOrder 1/2 (src/sim.cpp:47,56-66), stepmax, boundary codes xleft..zright 0..3
+ * RcOrder 1/2 (src/sim.cpp:47,56-66), stepmax, boundary codes xleft..zright: 0 copy, 1 periodic,
+ * 2 reflective, 3 outflow, 4 fixed (2D x and y faces; its data comes from gk_set_fixed_boundary)
  * (src/boundary/boundary.cpp:13-125), constant gravity (src/source_term/source.cpp:23-31),
  * precision 64 or 32 (the reference's Real, src/gaukuk.hpp:5-9). */
 typedef struct {
@@ -104,7 +106,8 @@ gk_status gk_upload_active(gk_ctx* ctx, const double* host);
 gk_status gk_download_active(gk_ctx* ctx, double* host);
 
 /* Sim::Advance(dtoutput) (src/sim.cpp:69-117): step until t reaches t + dtoutput or stepmax.
- * On return *t and *step hold the new time and step count (either may be NULL). */
+ * On return *t and *step hold the new time and step count (either may be NULL).  Steps are
+ * enqueued in chunks sized from the steps left at the current dt (at most 256 per host check). */
 gk_status gk_advance(gk_ctx* ctx, double dtoutput, double* t, int* step);
 /* Sim::GetTime / Getdt / GetStep and isContinue (src/sim.hpp:41-43, :26). */
 gk_status gk_get_time(gk_ctx* ctx, double* t, double* dt, int* step, int* is_continue);
@@ -119,7 +122,8 @@ gk_status gk_step_log(gk_ctx* ctx, int* steps, double* t, double* dt, int cap, i
 /* Enqueue exactly nsteps steps with no output deadline (for benchmarking); asynchronous. */
 gk_status gk_run_steps(gk_ctx* ctx, int nsteps);
 gk_status gk_synchronize(gk_ctx* ctx);
-/* Time nsteps steps with HIP events on the context's stream (one event after every launch).
+/* Time nsteps steps with HIP events on the context's stream (one event after every launch; all
+ * events are created before the first launch).
  * total_ms: the whole region; kernel_ms[k] / launches[k]: summed duration and count of the
  * stage kernel at position k = 0, 1, 2 within the step, and k = 3 the one-thread clock kernels. */
 gk_status gk_profile_steps(gk_ctx* ctx, int nsteps, double* total_ms, double kernel_ms[4], int launches[4]);
@@ -224,7 +228,9 @@ gk_status gk_flux_download(gk_ctx* ctx, double* acc);
  * fine grid's time and restarts its step count). */
 gk_status gk_restrict(gk_ctx* fine, gk_ctx* coarse);
 /* Attach a network to a precision-32 PLM grid without gravity (NULL detaches): its stages then
- * run the corrected kernel.  The network must outlive the attachment. */
+ * run the corrected kernel.  The network must outlive the attachment.  While one is attached,
+ * gk_set_point_gravity, gk_set_fixed_boundary and gk_set_immersed_shell fail with GK_EINVAL, and
+ * stepping re-checks the configuration. */
 gk_status gk_set_net(gk_ctx* ctx, gk_net* net);
 /* Face counts since the last reset: corrected, shock-sensor fallbacks, positivity fallbacks. */
 gk_status gk_net_stats(gk_ctx* ctx, long long out[3], int reset);

@@ -219,6 +219,29 @@ def test_corrected_stage_zero_net_equals_plain(hip):
     assert corrected == 10 * 3 * 2 * 64 * 64
 
 
+def test_attached_net_rejects_unsupported_setters(hip):
+    """The corrected stage has no gravity kick and no saved-data boundaries: with a network
+    attached, enrolling gravity, fixed data or a shell fails instead of being silently dropped;
+    after detaching, the same calls succeed."""
+    prob = problems.kelvin_helmholtz(64, seed=6, integrator=3)
+    s = hip.Solver(prob, precision=32)
+    net = hip.Net(np.zeros(L.NPARAM), np.ones(16), np.ones(4))
+    s.set_net(net)
+    with pytest.raises(hip.GaukukError, match="GK_EINVAL"):
+        s._check(s.L.gk_set_point_gravity(s.h, 1.0, 0.5, 0.5, 0.0, 0.0, 0.0, 0.0, 0.1))
+    with pytest.raises(hip.GaukukError, match="GK_EINVAL"):
+        s.set_fixed_boundary(np.zeros(s._ref_shape()))
+    with pytest.raises(hip.GaukukError, match="GK_EINVAL"):
+        s._check(s.L.gk_set_immersed_shell(s.h, 0.5, 0.5, 0.0, 0.1))
+    s.run_steps(2)   # still runnable as configured
+    s.synchronize()
+    s.set_net(None)
+    s._check(s.L.gk_set_point_gravity(s.h, 1.0, 0.5, 0.5, 0.0, 0.0, 0.0, 0.0, 0.1))
+    # a grid that gained gravity cannot take the network back
+    with pytest.raises(hip.GaukukError, match="GK_EINVAL"):
+        s.set_net(net)
+
+
 @pytest.mark.parametrize("kappa", [0.5, 4.0])
 def test_corrected_step_matches_oracle(hip, kappa):
     n = 32
