@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,7 +31,21 @@ struct StageSpec {
     bool first, last;
 };
 
+// Blocks of the conservation-sum reduction: each writes five partial sums, added in block order.
+constexpr int kSumBlocks = 512;
+
 }  // namespace
+
+// Partial sums of gk_cons_sums_noalloc: one array per GPU in the library's device memory (20 KiB),
+// held until the process ends.  Host code never uses the name itself (a host-side stand-in, not a
+// device address): it uses the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[5 * kSumBlocks];
+
+// Per-device address of `scratch`, looked up once per device on the first call made with that
+// device current.
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {nullptr};
+static std::mutex g_scratch_mu;
 
 // How one stage launch divides the grid: segments of seg_rows rows, strips x segs (x planes) waves.
 struct WavePlan {
@@ -51,7 +66,7 @@ struct gk_ctx {
     int log_cap = 4096;
     double* partial = nullptr;
     double* sums = nullptr;
-    int sum_blocks = 512;
+    int sum_blocks = kSumBlocks;
     DevClock* hclock = nullptr;  // pinned mirror
     hipStream_t stream = nullptr;
     int seg_rows = 0, strips = 0, segs = 0, nwaves = 0, nblocks = 0;
@@ -1100,6 +1115,53 @@ gk_status gk_cons_sums(gk_ctx* c, double out[5]) {
     GK_TRY_HIP(c, hipGetLastError());
     GK_TRY_HIP(c, hipMemcpyAsync(out, c->sums, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    return GK_OK;
+}
+
+gk_status gk_cons_sums_noalloc(const void* state, int nx, int ny, int nz, int precision, double* out, void* stream) {
+    if (!state || !out || nx <= 0 || ny <= 0 || nz <= 0 || (precision != 64 && precision != 32)) {
+        g_create_error = "gk_cons_sums_noalloc: null pointer, non-positive size or precision not 64/32";
+        return GK_EINVAL;
+    }
+    // the 2560 partials do not depend on the input's size; the element count bounds the indices
+    if ((long long)nx * ny * nz >= (1LL << 40)) {
+        g_create_error = "gk_cons_sums_noalloc: state larger than 2^40 cells";
+        return GK_EINVAL;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        g_create_error = "gk_cons_sums_noalloc: no current device (or its index exceeds 63)";
+        return GK_EHIP;
+    }
+    double* partial = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_dev[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                g_create_error = std::string("gk_cons_sums_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e);
+                return GK_EHIP;
+            }
+            g_scratch_dev[dev] = static_cast<double*>(p);
+        }
+        partial = g_scratch_dev[dev];
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const long long plane = (long long)nx * ny * nz;
+    // the kernels of gk_cons_sums: every block writes its five partials, so nothing needs clearing
+    if (precision == 32)
+        hipLaunchKernelGGL(sums_partial_kernel<float>, dim3(kSumBlocks), dim3(256), 0, st, (const float*)state, plane,
+                           nx, nx, ny * nz, partial);
+    else
+        hipLaunchKernelGGL(sums_partial_kernel<double>, dim3(kSumBlocks), dim3(256), 0, st, (const double*)state,
+                           plane, nx, nx, ny * nz, partial);
+    hipLaunchKernelGGL(sums_final_kernel, dim3(1), dim3(64), 0, st, partial, kSumBlocks, out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_create_error = std::string("gk_cons_sums_noalloc: launch: ") + hipGetErrorString(e);
+        return GK_EHIP;
+    }
     return GK_OK;
 }
 

@@ -20,7 +20,7 @@ GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ES
 SYMBOLS = ["gk_set_time", "gk_set_point_gravity", "gk_set_fixed_boundary", "gk_set_immersed_shell", "gk_update_bd", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
-           "gk_get_tile", "gk_selftest", "gk_last_error", "gk_version",
+           "gk_get_tile", "gk_selftest", "gk_cons_sums_noalloc", "gk_last_error", "gk_version",
            "gk_ens_create", "gk_ens_destroy", "gk_ens_upload", "gk_ens_download", "gk_ens_set_tile",
            "gk_ens_get_tile", "gk_ens_run_steps", "gk_ens_capture", "gk_ens_synchronize", "gk_ens_time",
            "gk_ens_get_time", "gk_ens_last_error",
@@ -93,6 +93,8 @@ def lib() -> C.CDLL:
         L.gk_profile_steps.argtypes = [vp, ip, dp, dp, C.POINTER(ip)]
         L.gk_cons_sums.restype = st
         L.gk_cons_sums.argtypes = [vp, dp]
+        L.gk_cons_sums_noalloc.restype = st
+        L.gk_cons_sums_noalloc.argtypes = [vp, ip, ip, ip, ip, vp, vp]
         L.gk_check_finite.restype = st
         L.gk_check_finite.argtypes = [vp]
         L.gk_set_tile.restype = st

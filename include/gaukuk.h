@@ -130,6 +130,18 @@ gk_status gk_profile_steps(gk_ctx* ctx, int nsteps, double* total_ms, double ker
 
 /* FP64 sums of the five conserved variables over active cells, deterministic. */
 gk_status gk_cons_sums(gk_ctx* ctx, double out[5]);
+/* gk_cons_sums without allocating (no counterpart in the reference: for callers that cannot
+ * allocate device memory, such as code under HIP graph capture or a tight serving loop).  The same
+ * kernels and summation order as gk_cons_sums, so the same bits, on a state the caller holds in
+ * device memory: `state` is [5][nz][ny][nx] (x fastest) of double (precision 64) or float (32);
+ * `out` receives 5 doubles in device memory; `stream` is a hipStream_t (NULL: the null stream);
+ * the call only enqueues work.  The partial sums live in a 20 KiB array of the library's device
+ * memory, one per GPU, held until the process ends (its size does not depend on the state's);
+ * every call overwrites all of it, and calls that overlap on one GPU race on it.  Calls no
+ * hipMalloc or hipFree.  Errors: GK_EINVAL (null pointer, non-positive size, precision not 64/32,
+ * more than 2^40 cells), GK_EHIP (no current device, symbol lookup or launch failed); the text is
+ * in gk_last_error(NULL). */
+gk_status gk_cons_sums_noalloc(const void* state, int nx, int ny, int nz, int precision, double* out, void* stream);
 /* GK_ENUMERIC if any active conserved value is NaN or infinite. */
 gk_status gk_check_finite(gk_ctx* ctx);
 
