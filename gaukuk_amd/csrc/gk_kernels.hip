@@ -410,13 +410,26 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 #pragma unroll
         for (int n = 0; n < 5; n++) wLy[n] = wLn[n];
     }
+    // FP32: the combine operands of the next update row are loaded one row ahead, a whole row
+    // iteration before their use (10 more registers, within what the FP32 kernels' occupancy
+    // leaves free; FP64 would lose a resident wave per SIMD).  Past the segment the row index is
+    // clamped, as for the prefetched input row.
+    constexpr bool EARLY_OPS = sizeof(R) == 4;
+    R nbs[5], nut[5];
+    if (EARLY_OPS) load_operands(j0 + 2, nbs, nut);   // row j0, the first update
     // steady state at newest row r: y face r-1 and the update of row r-2 (primitives in A)
     auto body = [&](int r, R (&A)[5], R (&B)[5], R (&Cn)[5]) {
         R u[5], bs[5], ut[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) u[n] = pre[n];
         load_row(min(r + 1, rlast), pre);
-        load_operands(r, bs, ut);
+        if (EARLY_OPS) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) { bs[n] = nbs[n]; ut[n] = nut[n]; }
+            load_operands(min(r + 1, rlast), nbs, nut);
+        } else {
+            load_operands(r, bs, ut);
+        }
         finish_row(r, u, Cn);
         R wR[5], wLn[5], G[5], F[5], Fn[5];
         yslope(A, B, Cn, wR, wLn);

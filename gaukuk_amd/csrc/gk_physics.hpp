@@ -18,6 +18,23 @@ namespace gk {
 
 template <class R> __host__ __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
 template <class R> __host__ __device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
+// FP32 on the device (judged by L1 against FP64, not bit for bit): one v_min_f32 / v_max_f32
+// instead of a compare and a select; the results differ from std::min/std::max only for a NaN
+// operand and in the sign of a zero result.
+template <> __host__ __device__ __forceinline__ float smin<float>(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fminf(a, b);
+#else
+    return (b < a) ? b : a;
+#endif
+}
+template <> __host__ __device__ __forceinline__ float smax<float>(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fmaxf(a, b);
+#else
+    return (a < b) ? b : a;
+#endif
+}
 
 template <class R> __host__ __device__ __forceinline__ R gsqrt(R x);
 template <> __host__ __device__ __forceinline__ double gsqrt<double>(double x) { return __builtin_sqrt(x); }
@@ -80,7 +97,58 @@ __host__ __device__ __forceinline__ double qsqrt(double x) {
 }
 __host__ __device__ __forceinline__ float qsqrt(float x) { return __builtin_sqrtf(x); }
 
+// Quotients and roots of the stencil arithmetic.  FP64 keeps the exact IEEE results above (the
+// FP64 path is compared with the reference at 1e-12).  FP32 is judged by its L1 distance from the
+// FP64 run, not bit for bit, so it uses the hardware reciprocal, square root and reciprocal square
+// root (v_rcp_f32, v_sqrt_f32, v_rsq_f32: about 1 ulp) in place of the IEEE sequences (8-12
+// instructions each); operands here are normal numbers (floored densities and pressures, sums of
+// them).  The host build uses the plain operators.
+__host__ __device__ __forceinline__ double srcp(double d) { return qdiv(1.0, d); }
+__host__ __device__ __forceinline__ double sdiv(double n, double d) { return qdiv(n, d); }
+__host__ __device__ __forceinline__ double ssqrt(double x) { return qsqrt(x); }
+__host__ __device__ __forceinline__ float srcp(float d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(d);
+#else
+    return 1.0f / d;
+#endif
+}
+__host__ __device__ __forceinline__ float sdiv(float n, float d) { return n * srcp(d); }
+__host__ __device__ __forceinline__ float ssqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
+// sqrt(a / b) for positive normal a, b: FP64 the IEEE root of the IEEE quotient (the reference's
+// SoundSpeed order); FP32 a * rsq(a * b), one hardware reciprocal square root.
+__host__ __device__ __forceinline__ double sqrt_ratio(double a, double b) { return qsqrt(qdiv(a, b)); }
+__host__ __device__ __forceinline__ float sqrt_ratio(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * __builtin_amdgcn_rsqf(a * b);
+#else
+    return __builtin_sqrtf(a / b);
+#endif
+}
+// 1 / sqrt(x): FP64 the IEEE quotient of the IEEE root; FP32 the hardware reciprocal square root.
+__host__ __device__ __forceinline__ double srsqrt(double x) { return qdiv(1.0, qsqrt(x)); }
+__host__ __device__ __forceinline__ float srsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / __builtin_sqrtf(x);
+#endif
+}
+
 template <class R> __host__ __device__ __forceinline__ R gabs(R x) { return x < R(0) ? -x : x; }
+template <> __host__ __device__ __forceinline__ float gabs<float>(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fabsf(x);
+#else
+    return x < 0.0f ? -x : x;
+#endif
+}
 
 // Density and pressure floors: GAUKUK_DENSITY_FLOOR / GAUKUK_PRESSURE_FLOOR (src/gaukuk.hpp:11-21).
 #ifndef GK_DENSITY_FLOOR
@@ -104,13 +172,13 @@ struct Adiabatic {
         e.gmRec = R(1.0) / e.gamma;
         return e;
     }
-    __host__ __device__ __forceinline__ R sound_speed(R d, R p) const { return qsqrt(qdiv(gamma * p, d)); }
+    __host__ __device__ __forceinline__ R sound_speed(R d, R p) const { return sqrt_ratio(gamma * p, d); }
     __host__ __device__ __forceinline__ R egas(R, R p) const { return p * gm1Rec; }
 
     // ConsToPrim with the floors clamped on local copies: adiabatic.cpp:18-61.
     __host__ __device__ __forceinline__ void cons_to_prim(const R u[5], R w[5]) const {
         R d = smax(u[0], R(GK_DENSITY_FLOOR));
-        R inv = qdiv(R(1.0), d);
+        R inv = srcp(d);
         w[0] = d;
         w[1] = u[1] * inv;
         w[2] = u[2] * inv;
@@ -122,7 +190,7 @@ struct Adiabatic {
 
     // One cell's max_d |v_d| + c_s with CalCmax's own kinetic-energy form: adiabatic.cpp:115-160.
     __host__ __device__ __forceinline__ R wave_speed(const R u[5]) const {
-        return wave_speed(u, qdiv(R(1.0), smax(u[0], R(GK_DENSITY_FLOOR))));
+        return wave_speed(u, srcp(smax(u[0], R(GK_DENSITY_FLOOR))));
     }
     // The same with inv = 1 / max(u[0], floor) already formed (floors() returns it).
     __host__ __device__ __forceinline__ R wave_speed(const R u[5], R inv) const {
@@ -130,7 +198,7 @@ struct Adiabatic {
         R ek = R(0.5) * (v1 * u[1] + v2 * u[2] + v3 * u[3]);
         R p = gm1 * (u[4] - ek);
         p = (p > R(GK_PRESSURE_FLOOR)) ? p : R(GK_PRESSURE_FLOOR);
-        R cs = qsqrt(gamma * p * inv);
+        R cs = ssqrt(gamma * p * inv);
         R loc = gabs(v1) + cs;
         loc = smax(loc, gabs(v2) + cs);
         loc = smax(loc, gabs(v3) + cs);
@@ -146,7 +214,7 @@ struct Adiabatic {
         const R dmin = R(GK_DENSITY_FLOOR), pmin = R(GK_PRESSURE_FLOOR);
         const R pm_gm1 = pmin / gm1;
         const bool low_d = r[0] < dmin;
-        const R inv = qdiv(R(1.0), r[0]);
+        const R inv = srcp(r[0]);
         const R ke = R(0.5) * inv * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
         const R press = gm1 * (r[4] - ke);
         const R e_fix = (press < pmin) ? pm_gm1 + ke : r[4];
@@ -167,7 +235,7 @@ struct VanLeer {
     static constexpr int order = 2;
     template <class R> __host__ __device__ __forceinline__ static R slope(R a, R b) {
         R multi = a * b;
-        R sigma = qdiv(R(2.0) * multi, a + b);
+        R sigma = sdiv(R(2.0) * multi, a + b);
         return (multi <= R(0.0)) ? R(0.0) : sigma;
     }
 };
@@ -191,13 +259,13 @@ struct Hllc {
         R pv = R(0.5) * (prel + prer) - R(0.125) * (vxr - vxl) * (denl + denr) * (al + ar);
         pv = smax(pv, R(0));
         const R qk = R(0.5) * (R(1) + eos.gmRec);
-        R tmp = smax(qdiv(pv, prel) - R(1), R(0));
-        const R ql = qsqrt(R(1) + qk * tmp);
-        tmp = smax(qdiv(pv, prer) - R(1), R(0));
-        const R qr = qsqrt(R(1) + qk * tmp);
+        R tmp = smax(sdiv(pv, prel) - R(1), R(0));
+        const R ql = ssqrt(R(1) + qk * tmp);
+        tmp = smax(sdiv(pv, prer) - R(1), R(0));
+        const R qr = ssqrt(R(1) + qk * tmp);
         const R sl = vxl - al * ql;
         const R sr = vxr + ar * qr;
-        const R ss = qdiv(prer - prel + denl * vxl * (sl - vxl) - denr * vxr * (sr - vxr),
+        const R ss = sdiv(prer - prel + denl * vxl * (sl - vxl) - denr * vxr * (sr - vxr),
                           denl * (sl - vxl) - denr * (sr - vxr));
         // Only the upwind side of the contact enters the flux: the reference blends
         // F = sel*F_L + (1-sel)*F_R with sel in {0,1} (hllc.cpp:64-83), which equals the
@@ -210,7 +278,7 @@ struct Hllc {
         const bool take = (left & (sk < R(0))) | ((!left) & (R(0) < sk));
         const R mk = take ? sk : R(0);
         const R ek = eos.egas(dk, pk) + R(0.5) * dk * (vk * vk + v2 * v2 + v3 * v3);
-        const R ck = qdiv(mk * (ss - vk), sk - ss);
+        const R ck = sdiv(mk * (ss - vk), sk - ss);
         const R f1 = dk * (vk + ck);
         f[0] = f1;
         f[1] = dk * (vk * vk + ck * sk) + pk;
@@ -234,7 +302,7 @@ __host__ __device__ __forceinline__ void bc_transform(R u[5], int comp, int kind
         const R m = u[comp];
         const R mc = (kind == 2) ? smin(m, R(0.0)) : smax(m, R(0.0));
         const R d = smax(u[0], R(GK_DENSITY_FLOOR));
-        const R inv = qdiv(R(1.0), d);
+        const R inv = srcp(d);
         const R keOld = R(0.5) * inv * m * m;
         const R keNew = R(0.5) * inv * mc * mc;
         R eint = u[4] - keOld;
@@ -252,7 +320,7 @@ __host__ __device__ __forceinline__ void bc_transform_lanes(R u[5], int comp, in
     const R m = u[comp];
     const R mc = (kind == 2) ? smin(m, R(0.0)) : smax(m, R(0.0));
     const R d = smax(u[0], R(GK_DENSITY_FLOOR));
-    const R inv = qdiv(R(1.0), d);
+    const R inv = srcp(d);
     const R keOld = R(0.5) * inv * m * m;
     const R keNew = R(0.5) * inv * mc * mc;
     R eint = u[4] - keOld;
@@ -309,7 +377,7 @@ __host__ __device__ __forceinline__ void point_kick(R u[5], int npoint, const do
     const R gm0dt = R(m0[0]) * h, rs0 = R(m0[4]);
     const R delx0 = x - R(m0[1]), dely0 = y - R(m0[2]), delz0 = z - R(m0[3]);
     const R r0Sqr = delx0 * delx0 + dely0 * dely0 + delz0 * delz0;
-    const R r0SmoInv = qdiv(R(1.0), qsqrt(r0Sqr + rs0 * rs0));
+    const R r0SmoInv = srsqrt(r0Sqr + rs0 * rs0);
     const R g0 = -gm0dt * r0SmoInv * r0SmoInv * r0SmoInv;
     if (npoint == 1) {
         gxdt = g0 * delx0;
@@ -319,7 +387,7 @@ __host__ __device__ __forceinline__ void point_kick(R u[5], int npoint, const do
         const R gm1dt = R(m1[0]) * h, rs1 = R(m1[4]), z1 = R(m1[3]);
         const R delx1 = x - x1, dely1 = y - y1, delz1 = z - z1;
         const R r1Sqr = delx1 * delx1 + dely1 * dely1 + delz1 * delz1;
-        const R r1SmoInv = qdiv(R(1.0), qsqrt(r1Sqr + rs1 * rs1));
+        const R r1SmoInv = srsqrt(r1Sqr + rs1 * rs1);
         const R g1 = -gm1dt * r1SmoInv * r1SmoInv * r1SmoInv;
         const R gIndir = R(0);
         gxdt = g0 * delx0 + g1 * delx1 + gIndir * x1;
