@@ -109,6 +109,13 @@ struct gk_ctx {
     long long* sh_tar = nullptr;
     void* sh_dir = nullptr;
     void* sh_gam = nullptr;
+    // boundary hook (gk_set_bd_hook): faces whose ghost layers a host function refreshes before
+    // every stage, from a mirror of the stage input's edge bands
+    gk_bd_hook hook = nullptr;
+    void* hook_user = nullptr;
+    void* hook_mirror = nullptr;   // caller's reference-layout host array, state precision
+    int hook_mask = 0, hook_band = 0;
+    std::vector<char> hook_stage;  // staging of one face's band or ghost layers
     // learned flux correction (gk_learned.hip)
     gk_net* net = nullptr;                 // attached network: stages use the corrected kernel
     unsigned long long* lstats = nullptr;  // corrected / sensor / positivity face counts
@@ -137,6 +144,13 @@ static gk_status fail(gk_ctx* c, gk_status s, const std::string& m) {
 static bool has_fixed_face(const gk_ctx* c) {
     return c->bc[0] == 4 || c->bc[1] == 4 || c->bc[2] == 4 || c->bc[3] == 4;
 }
+static bool hooked(const gk_ctx* c, int f) { return c->hook && ((c->hook_mask >> f) & 1); }
+// type-4 faces whose data must come from gk_set_fixed_boundary (not refreshed by a hook)
+static bool has_enrolled_face(const gk_ctx* c) {
+    for (int f = 0; f < 4; f++)
+        if (c->bc[f] == 4 && !hooked(c, f)) return true;
+    return false;
+}
 // boundaries beyond the face remaps: fixed faces, the inner fixed region, an immersed shell
 static bool has_extra_boundaries(const gk_ctx* c) { return has_fixed_face(c) || c->inner || c->immersed; }
 
@@ -157,7 +171,7 @@ static gk_status check_fixed(gk_ctx* c) {
     // an attached network runs every stage: the configuration must still be one it can run
     if (c->net)
         if (const char* m = net_conflict(c)) return fail(c, GK_EINVAL, m);
-    if (has_fixed_face(c) && c->fixed_ref.empty())
+    if (has_enrolled_face(c) && c->fixed_ref.empty())
         return fail(c, GK_ESTATE,
                     "fixed boundary is not initialized: call gk_set_fixed_boundary (EnrollFixedBoundaryData)");
     // the shell is rewritten from the stage input in memory, while the first gravity half-kick is
@@ -461,6 +475,7 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
         for (int j = g; j < g + ny; j++)
             for (int i = 0; i < lx; i++) {
                 if (i >= g && i < g + nx) continue;
+                if (hooked(c, i < g ? 0 : 1)) continue;   // filled by the hook (gk_download_cons)
                 int kind;
                 const int si = bc_remap(i - g, nx, c->bc[0], c->bc[1], kind) + g;
                 R u[5];
@@ -471,6 +486,7 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
         if (ny > 1 || nz > 1)
             for (int j = 0; j < ly; j++) {
                 if (j >= g && j < g + ny) continue;
+                if (hooked(c, j < g ? 2 : 3)) continue;
                 for (int i = g; i < g + nx; i++) {
                     int kind;
                     const int sj = bc_remap(j - g, ny, c->bc[2], c->bc[3], kind) + g;
@@ -496,6 +512,144 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
                     for (int n = 0; n < 5; n++) at(n, k, j, i) = u[n];
                 }
         }
+}
+
+// ---- boundary hook (gk_set_bd_hook) ------------------------------------------------------
+// Face f's band of `band` active layers: x faces columns [c0, c0 + band) of every row, y faces
+// rows [r0, r0 + band) of every column.
+static void hook_band_origin(const gk_ctx* c, int f, int& c0, int& r0) {
+    const int b = c->hook_band;
+    c0 = f == 1 ? c->nx - b : 0;
+    r0 = f == 3 ? c->ny - b : 0;
+}
+
+// Write the bands of the hooked faces of `state` (device, [5][ny][nx] of R) into the mirror at
+// their reference positions; with `kick`, after the constant-gravity half-kick of length h.
+template <class R>
+static gk_status hook_write_bands(gk_ctx* c, const void* state, bool kick, double h) {
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, b = c->hook_band;
+    const int lx = nx + 2 * g, ly = ny + 2 * g;
+    R* mir = static_cast<R*>(c->hook_mirror);
+    const R* dev = static_cast<const R*>(state);
+    const R gx = R(c->phys.g[0]), gy = R(c->phys.g[1]), gz = R(c->phys.g[2]), hh = R(h);
+    for (int f = 0; f < 4; f++) {
+        if (!hooked(c, f)) continue;
+        int c0, r0;
+        hook_band_origin(c, f, c0, r0);
+        const bool xf = f < 2;
+        const int w = xf ? b : nx, rows = xf ? ny : b;
+        c->hook_stage.resize(sizeof(R) * 5 * (size_t)w * rows);
+        R* st = reinterpret_cast<R*>(c->hook_stage.data());
+        for (int n = 0; n < 5; n++)
+            GK_TRY_HIP(c, hipMemcpy2DAsync(st + (size_t)n * w * rows, sizeof(R) * w,
+                                           dev + (size_t)n * c->plane + (size_t)r0 * c->pitch + c0, sizeof(R) * c->pitch,
+                                           sizeof(R) * w, rows, hipMemcpyDeviceToHost, c->stream));
+        GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+        for (int j = 0; j < rows; j++)
+            for (int i = 0; i < w; i++) {
+                R u[5];
+                for (int n = 0; n < 5; n++) u[n] = st[((size_t)n * rows + j) * w + i];
+                if (kick) gravity_kick(u, gx, gy, gz, hh);
+                for (int n = 0; n < 5; n++)
+                    mir[(((size_t)n * (1 + 2 * g) + g) * ly + (g + r0 + j)) * lx + (g + c0 + i)] = u[n];
+            }
+    }
+    return GK_OK;
+}
+
+// The ghost layer at distance d (0 nearest) from hooked face f, as the mirror holds it, is
+// (n, t) -> value with t the tangential active index.
+template <class R>
+static R hook_ghost(const gk_ctx* c, int f, int n, int d, int t) {
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
+    const int lx = nx + 2 * g, ly = ny + 2 * g;
+    const R* mir = static_cast<const R*>(c->hook_mirror);
+    int i, j;
+    if (f < 2) { i = f == 0 ? g - 1 - d : g + nx + d; j = g + t; }
+    else { j = f == 2 ? g - 1 - d : g + ny + d; i = g + t; }
+    return mir[(((size_t)n * (1 + 2 * g) + g) * ly + j) * lx + i];
+}
+
+// Before stage q: mirror the stage input's bands, call the hook, upload the two ghost layers
+// nearest each hooked face into that face's fixed-face array (layouts of gk_set_fixed_boundary:
+// x faces [5][ny][2], y faces [5][2][nx]).
+template <class R>
+static gk_status hook_refresh_t(gk_ctx* c, size_t q) {
+    const StageSpec& s = c->stages[q];
+    // forward Euler alternates its two buffers with the clock's `cur`
+    const bool swap = c->phys.integrator == 1 && c->hclock->cur;
+    const void* in = c->buf[swap ? s.out : s.in];
+    if (gk_status st = hook_write_bands<R>(c, in, s.first && c->gravity, 0.5 * c->hclock->dt)) return st;
+    c->hook(c->hook_user, c->hook_mirror);
+    for (int f = 0; f < 4; f++) {
+        if (!hooked(c, f)) continue;
+        const int len = f < 2 ? c->ny : c->nx;
+        c->hook_stage.resize(sizeof(R) * 10 * (size_t)len);
+        R* st = reinterpret_cast<R*>(c->hook_stage.data());
+        for (int n = 0; n < 5; n++)
+            for (int d = 0; d < 2; d++)
+                for (int t = 0; t < len; t++) {
+                    const R v = hook_ghost<R>(c, f, n, d, t);
+                    if (!std::isfinite((double)v))
+                        return fail(c, GK_EINVAL, "the boundary hook left a non-finite ghost value on face " +
+                                                      std::to_string(f) + " (does the face function read cells "
+                                                      "beyond the mirrored band?)");
+                    st[f < 2 ? ((size_t)n * len + t) * 2 + d : ((size_t)n * 2 + d) * len + t] = v;
+                }
+        GK_TRY_HIP(c, hipMemcpyAsync(c->fix[f], st, sizeof(R) * 10 * (size_t)len, hipMemcpyHostToDevice, c->stream));
+        GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return GK_OK;
+}
+static gk_status hook_refresh(gk_ctx* c, size_t q) {
+    return c->prec == 32 ? hook_refresh_t<float>(c, q) : hook_refresh_t<double>(c, q);
+}
+
+// Output ghost cells of the hooked faces: the hook applied to the current state; all nGhost
+// layers over the active tangential range, as the mirror holds them after the call.
+template <class R>
+static gk_status hook_fill_output(gk_ctx* c, const void* state, R* ref) {
+    if (gk_status st = hook_write_bands<R>(c, state, false, 0.0)) return st;
+    c->hook(c->hook_user, c->hook_mirror);
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
+    const int lx = nx + 2 * g, ly = ny + 2 * g;
+    const R* mir = static_cast<const R*>(c->hook_mirror);
+    const size_t plane_ref = (size_t)(1 + 2 * g) * ly * lx;
+    auto copy = [&](int j, int i) {
+        for (int n = 0; n < 5; n++) {
+            const size_t o = n * plane_ref + ((size_t)g * ly + j) * lx + i;
+            ref[o] = mir[o];
+        }
+    };
+    for (int f = 0; f < 4; f++) {
+        if (!hooked(c, f)) continue;
+        for (int d = 0; d < g; d++) {
+            if (f < 2)
+                for (int j = g; j < g + ny; j++) copy(j, f == 0 ? g - 1 - d : g + nx + d);
+            else
+                for (int i = g; i < g + nx; i++) copy(f == 2 ? g - 1 - d : g + ny + d, i);
+        }
+    }
+    return GK_OK;
+}
+
+// Whole steps with the host back in the loop before every stage (contexts with a boundary hook):
+// the same kernels as enqueue_steps, one step at a time; stops early when the clock goes inactive.
+static gk_status hooked_steps(gk_ctx* c, int n) {
+    const int toggle = c->phys.integrator == 1 ? 1 : 0;
+    for (int k = 0; k < n; k++) {
+        hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock);
+        if (gk_status s = read_clock(c)) return s;
+        if (!c->hclock->active) break;
+        for (size_t q = 0; q < c->stages.size(); q++) {
+            if (gk_status s = hook_refresh(c, q)) return s;
+            launch_immersed(c, c->stages[q]);
+            launch_stage_any(c, c->stages[q]);
+        }
+        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, c->log, c->log_cap, toggle);
+        GK_TRY_HIP(c, hipGetLastError());
+    }
+    return GK_OK;
 }
 
 template <class T>
@@ -744,6 +898,11 @@ gk_status gk_download_cons(gk_ctx* c, void* host) {
     if (s != GK_OK) return s;
     if (c->prec == 32) active_to_ref(c, act.data(), (float*)host);
     else active_to_ref(c, act.data(), (double*)host);
+    if (c->hook) {
+        const void* cur = c->buf[current_buffer(c)];
+        return c->prec == 32 ? hook_fill_output<float>(c, cur, (float*)host)
+                             : hook_fill_output<double>(c, cur, (double*)host);
+    }
     return GK_OK;
 }
 
@@ -757,7 +916,11 @@ gk_status gk_advance(gk_ctx* c, double dtoutput, double* t, int* step) {
     if (s != GK_OK) return s;
     int last_step = -1;
     while (c->hclock->active) {
-        enqueue_steps(c, advance_chunk(*c->hclock));
+        if (c->hook) {
+            if ((s = hooked_steps(c, advance_chunk(*c->hclock))) != GK_OK) return s;
+        } else {
+            enqueue_steps(c, advance_chunk(*c->hclock));
+        }
         GK_TRY_HIP(c, hipGetLastError());
         if ((s = read_clock(c)) != GK_OK) return s;
         if (!c->hclock->active) break;
@@ -815,6 +978,7 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
                                 double rs) {
     if (!c) return GK_EINVAL;
     if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
+    if (c->hook) return fail(c, GK_EINVAL, "boundary hooks do not take point gravity");
     // SourceTerm::EnrollPointGravity (src/source_term/source.cpp:33-67): the first mass gives point
     // gravity, a second turns it into binary gravity, a third is an error
     if (c->npoint >= 2) return fail(c, GK_EINVAL, "Setup failed: too many point gravity.");
@@ -991,6 +1155,31 @@ gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, doub
     return GK_OK;
 }
 
+gk_status gk_set_bd_hook(gk_ctx* c, int face_mask, int band, gk_bd_hook hook, void* user, void* mirror) {
+    if (!c) return GK_EINVAL;
+    if (!hook || !mirror) return fail(c, GK_EINVAL, "gk_set_bd_hook: NULL hook or mirror");
+    if (c->nz > 1) return fail(c, GK_EINVAL, "boundary hooks are 2D only");
+    if (face_mask <= 0 || face_mask > 15) return fail(c, GK_EINVAL, "gk_set_bd_hook: face_mask selects faces 0-3");
+    if ((face_mask & 12) && c->ny < 2) return fail(c, GK_EINVAL, "gk_set_bd_hook: y faces of a one-row grid");
+    if (band < 1 || band > c->nx || band > c->ny)
+        return fail(c, GK_EINVAL, "gk_set_bd_hook: band must lie in [1, min(nx, ny)]");
+    if (c->npoint) return fail(c, GK_EINVAL, "boundary hooks do not take point gravity");
+    if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
+    GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    for (int f = 0; f < 4; f++) {
+        if (!((face_mask >> f) & 1)) continue;
+        if (!c->fix[f]) GK_TRY_HIP(c, hipMalloc(&c->fix[f], c->esz * 10 * (size_t)(f < 2 ? c->ny : c->nx)));
+        c->bc[f] = 4;
+    }
+    c->hook = hook;
+    c->hook_user = user;
+    c->hook_mirror = mirror;
+    c->hook_mask = face_mask;
+    c->hook_band = band;
+    c->plans.clear();   // the fixed-face kernel variant runs from now on
+    return GK_OK;
+}
+
 gk_status gk_update_bd(gk_ctx* c) {
     if (!c) return GK_EINVAL;
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
@@ -1026,7 +1215,11 @@ gk_status gk_run_steps(gk_ctx* c, int nsteps) {
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
     if (gk_status fs = check_fixed(c)) return fs;
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
-    enqueue_steps(c, nsteps);
+    if (c->hook) {
+        if (gk_status s = hooked_steps(c, nsteps)) return s;
+    } else {
+        enqueue_steps(c, nsteps);
+    }
     GK_TRY_HIP(c, hipGetLastError());
     return GK_OK;
 }
@@ -1040,6 +1233,7 @@ gk_status gk_synchronize(gk_ctx* c) {
 gk_status gk_profile_steps(gk_ctx* c, int nsteps, double* total_ms, double kernel_ms[4], int launches[4]) {
     if (!c || nsteps < 0) return GK_EINVAL;
     if (!c->uploaded) return fail(c, GK_ESTATE, "no state uploaded");
+    if (c->hook) return fail(c, GK_EINVAL, "per-launch profiling is not available with a boundary hook");
     if (gk_status fs = check_fixed(c)) return fs;
     hipLaunchKernelGGL(advance_begin_kernel, dim3(1), dim3(1), 0, c->stream, c->clock, 1e300);
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));

@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get("GK_LIB_PATH") or os.path.join(HERE, "libgaukuk_hip.so
 GK_STATUS = {0: "GK_OK", 1: "GK_EINVAL", 2: "GK_EHIP", 3: "GK_ENOMEM", 4: "GK_ESTATE", 5: "GK_ENUMERIC"}
 
 # Every symbol include/gaukuk.h declares (checked by tests/test_capi_symbols.py).
-SYMBOLS = ["gk_set_time", "gk_set_point_gravity", "gk_set_fixed_boundary", "gk_set_immersed_shell", "gk_update_bd", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
+SYMBOLS = ["gk_set_bd_hook", "gk_set_time", "gk_set_point_gravity", "gk_set_fixed_boundary", "gk_set_immersed_shell", "gk_update_bd", "gk_create", "gk_destroy", "gk_upload_cons", "gk_download_cons", "gk_upload_active",
            "gk_download_active", "gk_advance", "gk_get_time", "gk_step_log", "gk_run_steps",
            "gk_synchronize", "gk_profile_steps", "gk_cons_sums", "gk_check_finite", "gk_set_tile",
            "gk_get_tile", "gk_selftest", "gk_cons_sums_noalloc", "gk_last_error", "gk_version",
@@ -41,6 +41,10 @@ class GkPhys(C.Structure):
     _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("g", C.c_double * 3),
                 ("integrator", C.c_int), ("rc_order", C.c_int), ("stepmax", C.c_int),
                 ("bc", C.c_int * 6), ("precision", C.c_int)]
+
+
+# void hook(void* user, void* mirror): gk_set_bd_hook's host boundary function
+BD_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
 
 
 class GaukukError(RuntimeError):
@@ -71,6 +75,8 @@ def lib() -> C.CDLL:
         L.gk_set_immersed_shell.argtypes = [vp] + [C.c_double] * 4
         L.gk_set_time.restype = st
         L.gk_set_time.argtypes = [vp, C.c_double, ip]
+        L.gk_set_bd_hook.restype = st
+        L.gk_set_bd_hook.argtypes = [vp, ip, ip, BD_HOOK, vp, vp]
         L.gk_update_bd.restype = st
         L.gk_update_bd.argtypes = [vp]
         for name in ("gk_upload_cons", "gk_download_cons"):
@@ -283,6 +289,17 @@ class Solver:
         a = np.ascontiguousarray(ref, dt)
         assert a.shape == self._ref_shape()
         self._check(self.L.gk_set_fixed_boundary(self.h, a.ctypes.data, int(bool(fix_inner))))
+
+    def set_bd_hook(self, face_mask: int, band: int, fn):
+        """gk_set_bd_hook: faces in face_mask (bit f: 0 x left .. 3 y right) get their ghost layers
+        from fn(mirror) before every stage, mirror being the reference-layout array
+        [5][NZ+2g][NY+2g][NX+2g] in the solver's precision (NaN except the mirrored bands of
+        `band` active layers next to those faces, which the library writes before each call)."""
+        dt = np.float64 if self.precision == 64 else np.float32
+        self._mirror = np.full(self._ref_shape(), np.nan, dt)
+        self._hook_fn = BD_HOOK(lambda user, ptr: fn(self._mirror))
+        self._check(self.L.gk_set_bd_hook(self.h, face_mask, band, self._hook_fn, None,
+                                          self._mirror.ctypes.data))
 
     def update_bd(self):
         """Boundary::UpdateBD's in-place part on the current state (immersed shell, inner region),

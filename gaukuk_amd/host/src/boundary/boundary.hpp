@@ -32,8 +32,10 @@ public:
 
     // Self-defined faces (setup-file hooks, src/boundary/boundary.hpp:28-45): the setup file defines
     // the static member for a face and enrolls it by name, or passes any function.  On the device a
-    // self-defined face whose ghost values do not depend on the state (an inflow) runs as a fixed
-    // face; the Sim checks that on a perturbed state and rejects any other.
+    // self-defined face whose ghost values do not depend on the state (an inflow; the Sim checks
+    // that on a perturbed state) runs as a fixed face; any other 2D face runs through the boundary
+    // hook (gk_set_bd_hook), which calls the function before every stage on a mirror of the
+    // state's edge bands.
     static void SelfDefineBDXL(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
     static void SelfDefineBDXR(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
     static void SelfDefineBDYL(TArray<Real>& cons, const Grid& grid, const EquationOfState& eos);
@@ -66,8 +68,8 @@ public:
     void ClearDeviceUpdate() { pending_ = false; }
 
     // Fixed faces (src/boundary/fixed.cpp:242-269): keeps a copy of `cons`, whose ghost layers
-    // type-4 faces restore.  Inner fixed regions (fixInner, src/boundary/boundary.cpp:139-141) have
-    // no device implementation and are rejected.
+    // type-4 faces restore; with fixInner the inner region's active cells are restored before every
+    // stage as well (src/boundary/boundary.cpp:139-141, fixed.cpp:181-207; 2D on the device).
     void EnrollFixedBoundaryData(const TArray<Real>& cons, const Grid& grid, bool fixInner = false);
     bool AnyFixed() const;
     bool FixedEnrolled() const { return fixed_->enrolled; }

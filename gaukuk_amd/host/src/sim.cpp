@@ -5,6 +5,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <stdexcept>
 #include <vector>
 
@@ -59,9 +60,9 @@ void Sim::Check(gk_status s, const char* what) const {
 // A self-defined face runs on the device as a fixed face when the ghost values it writes do not
 // depend on the state: it is applied to the current state and to one with every active value
 // changed, and the ghost layers it fills (the face's nGhost layers over the active tangential
-// range, which are all the stage reads) must agree bit for bit.  Returns the layers in `out`.
-void Sim::ProbeSelfDefined(int f, TArray<Real>& out) const {
-    static const char* names[6] = {"x left", "x right", "y left", "y right", "z left", "z right"};
+// range, which are all the stage reads) must agree bit for bit.  Returns whether they do, with the
+// layers in `out`.  A face that depends on the state runs through the boundary hook instead.
+bool Sim::ProbeSelfDefined(int f, TArray<Real>& out) const {
     TArray<Real> a = cons, b = cons;
     for (int v = 0; v < NVar; v++)
         for (int k = grid.kb; k < grid.ke; k++)
@@ -76,11 +77,18 @@ void Sim::ProbeSelfDefined(int f, TArray<Real>& out) const {
         for (int k = from[2]; k < to[2]; k++)
             for (int j = from[1]; j < to[1]; j++)
                 for (int i = from[0]; i < to[0]; i++)
-                    if (std::memcmp(&a(v, k, j, i), &b(v, k, j, i), sizeof(Real)) != 0)
-                        throw std::runtime_error(std::string("the self-defined boundary on the ") + names[f] +
-                                                 " face depends on the state: only state-independent "
-                                                 "self-defined faces (e.g. inflows) run on the device");
+                    if (std::memcmp(&a(v, k, j, i), &b(v, k, j, i), sizeof(Real)) != 0) return false;
     out = a;
+    return true;
+}
+
+// gk_set_bd_hook's function: the enrolled face functions of the state-dependent self-defined
+// faces, in the reference's UpdateBD order (src/boundary/boundary.cpp:127-135), on the mirror
+// array, whose active cells outside the mirrored bands hold NaN.
+void Sim::BdHook(void* user, void*) {
+    Sim* s = static_cast<Sim*>(user);
+    for (int f = 0; f < 4; f++)
+        if (s->hookFaces_ & (1 << f)) s->boundary.Face(f)(s->hookMirror_, s->grid, s->eos);
 }
 
 void Sim::EnsureDevice() {
@@ -99,12 +107,22 @@ void Sim::EnsureDevice() {
         p.stepmax = stepMax;
         for (int f = 0; f < 6; f++) p.bc[f] = boundary.type[f];
         // state-independent self-defined faces become fixed faces holding the ghost values their
-        // function writes
+        // function writes; state-dependent ones (2D) are refreshed before every stage through the
+        // boundary hook
+        static const char* names[6] = {"x left", "x right", "y left", "y right", "z left", "z right"};
         const int nface = grid.nz > 1 ? 6 : 4;
         for (int f = 0; f < nface; f++) {
             if (!boundary.selfDefined[f]) continue;
             TArray<Real> ghosts;
-            ProbeSelfDefined(f, ghosts);
+            if (!ProbeSelfDefined(f, ghosts)) {
+                if (grid.nz > 1)
+                    throw std::runtime_error(std::string("the self-defined boundary on the ") + names[f] +
+                                             " face depends on the state: in 3D only state-independent "
+                                             "self-defined faces run on the device");
+                hookFaces_ |= 1 << f;
+                p.bc[f] = 4;
+                continue;
+            }
             if (selfGhosts_.GetSize() == 0) selfGhosts_ = ghosts;
             CopyFace(grid, f, ghosts, selfGhosts_);
             selfFaces_[f] = true;
@@ -118,6 +136,11 @@ void Sim::EnsureDevice() {
         for (int q = 0; q < srcTerm.nPoint; q++) {
             const Real* m = srcTerm.mass[q];
             Check(gk_set_point_gravity(ctx_, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7]), "gk_set_point_gravity");
+        }
+        if (hookFaces_) {
+            hookMirror_.NewArray(NVar, grid.lenz, grid.leny, grid.lenx);
+            for (size_t k = 0; k < hookMirror_.GetSize(); k++) hookMirror_.data()[k] = std::numeric_limits<Real>::quiet_NaN();
+            Check(gk_set_bd_hook(ctx_, hookFaces_, grid.nGhost, &Sim::BdHook, this, hookMirror_.data()), "gk_set_bd_hook");
         }
     }
     // fixed faces and the inner region: the enrolled snapshot (re-sent when the setup enrolls new

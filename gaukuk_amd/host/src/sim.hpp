@@ -70,7 +70,10 @@ private:
     bool shellSent_ = false;  // the immersed shell is set up on the device
     bool selfFaces_[6] = {false, false, false, false, false, false};  // self-defined faces run as fixed
     TArray<Real> selfGhosts_;  // the ghost layers those faces write
-    void ProbeSelfDefined(int f, TArray<Real>& out) const;
+    int hookFaces_ = 0;        // state-dependent self-defined faces, refreshed through the hook
+    TArray<Real> hookMirror_;  // the hook's mirror array (NaN outside the mirrored bands)
+    bool ProbeSelfDefined(int f, TArray<Real>& out) const;
+    static void BdHook(void* user, void* mirror);
     bool verbose_ = true;
     void EnsureDevice();
     void Check(gk_status s, const char* what) const;

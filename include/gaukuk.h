@@ -85,6 +85,24 @@ gk_status gk_set_fixed_boundary(gk_ctx* ctx, const void* cons, int fix_inner);
  * than 8 image cells); advancing with gravity enrolled fails with GK_EINVAL. */
 gk_status gk_set_immersed_shell(gk_ctx* ctx, double x, double y, double z, double r);
 
+/* Self-defined faces whose ghost values depend on the state (Boundary::EnrollSelfDefineBD*,
+ * src/boundary/boundary.hpp:28-45; the reference calls them in every UpdateBD, i.e. before every
+ * stage: src/evolution/rk2.cpp:12,17, boundary.cpp:127-135).  The faces in `face_mask` (bit f for
+ * face f: 0 x left, 1 x right, 2 y left, 3 y right; 2D grids) become fixed (type-4) faces whose
+ * two ghost layers nearest the face are refreshed before every stage: the library writes the
+ * `band` active layers nearest each of those faces of the stage's input (after the first gravity
+ * half-kick in the first stage, as the reference's face functions see it) into `mirror` at their
+ * reference-layout positions, calls hook(user, mirror), and reads the ghost layers back from
+ * `mirror`.  `mirror` is the caller's host array Real[5][lenz][leny][lenx] in the context's
+ * precision; the library writes only those bands, so cells the caller poisoned (e.g. with NaN)
+ * stay poisoned, and a face function that reads beyond the band yields non-finite ghosts, which
+ * fail the step with GK_EINVAL.  Stepping returns to the host once per stage; gk_download_cons
+ * fills the hooked faces' ghost cells by calling the hook on the current state.  Fails with
+ * GK_EINVAL for 3D grids, a NULL hook or mirror, band outside [1, min(nx, ny)], y faces of a
+ * one-row grid, point gravity or an attached network.  A later call replaces the hook. */
+typedef void (*gk_bd_hook)(void* user, void* mirror);
+gk_status gk_set_bd_hook(gk_ctx* ctx, int face_mask, int band, gk_bd_hook hook, void* user, void* mirror);
+
 /* The in-place part of Boundary::UpdateBD on the current state (src/boundary/boundary.cpp:127-142):
  * the immersed shell, then the inner fixed region.  The reference's main calls UpdateBD once after
  * Setup (src/main.cpp:30); callers that upload a state and advance it call this in between. */

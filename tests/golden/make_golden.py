@@ -236,6 +236,36 @@ SETUP_SIZES = {
 }
 
 
+# setup files of this project in the reference's plug-in format, built into the reference
+# (oracle/_ref/gaukuk_ref_<s>, `make -C oracle project-setups`) and run on these .in values
+PROJECT_SETUPS = {
+    # tests/golden/setup_statebd.cpp: state-dependent self-defined faces on all four sides
+    "statebd": dict(gamma=1.4, tmax=1e30, stepmax=60, CFL=0.4, integrator=3, RcOrder=2, dtoutput=1e30,
+                    dataType=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0, NX=80, NY=72, NZ=1,
+                    NGhost=2, xleft=0, xright=0, yleft=0, yright=0, zleft=0, zright=0, rho0=1.0, p0=1.0,
+                    p_ratio=20.0, r0=0.15, x0=0.3, y0=0.35, vshear=0.4, lidU=0.3),
+}
+
+
+def project_setups():
+    """This project's plug-in setups, built into the reference, on their .in values."""
+    for name, vals in PROJECT_SETUPS.items():
+        vals = dict(vals)
+        with tempfile.TemporaryDirectory() as d:
+            write_in(os.path.join(d, "case.in"), vals)
+            r = subprocess.run([os.path.join(REF, f"gaukuk_ref_{name}"), "-i", "case.in"], cwd=d,
+                               env=dict(os.environ, OMP_NUM_THREADS="4"), capture_output=True, text=True,
+                               check=True)
+            steps = int(re.search(r"total steps =\s*(\d+)", r.stdout).group(1))
+            kind = "prim" if int(vals.get("dataType", 0)) == 1 else "cons"
+            outs = sorted(f for f in os.listdir(d) if f.startswith(kind + "_"))
+            last = GaukukOutput(os.path.join(d, outs[-1]))
+            np.savez_compressed(os.path.join(HERE, f"setup_{name}.npz"), final=last.active().copy(),
+                                t=np.array(float(last.t)), steps=np.array(steps), kind=kind, setup=name,
+                                meta=json.dumps(vals))
+        print(f"project setup {name:10s} {vals['NX']}x{vals['NY']} steps={steps} t={float(last.t)!r} ({kind})")
+
+
 def reference_setups():
     """Each reference setup file, built into the reference itself, on its shipped input file."""
     for name, over in SETUP_SIZES.items():
@@ -284,7 +314,12 @@ def main():
     ap.add_argument("--only", default="", help="comma-separated run cases to (re)make; default: everything")
     ap.add_argument("--setups", action="store_true", help="(re)make only the reference-setup fixtures")
     ap.add_argument("--random", action="store_true", help="(re)make only the random-configuration checksums")
+    ap.add_argument("--project-setups", action="store_true",
+                    help="(re)make only the fixtures of this project's plug-in setups")
     a = ap.parse_args()
+    if a.project_setups:
+        project_setups()
+        return
     if a.setups:
         reference_setups()
         return

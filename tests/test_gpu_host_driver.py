@@ -108,12 +108,22 @@ def test_driver_solid_body_setup(tmp_path, binary):
     assert np.all(err <= 1e-12), err
 
 
-def test_driver_rejects_state_dependent_self_face(tmp_path):
+def test_driver_state_dependent_self_face(tmp_path):
+    """A self-defined x-left face that copies the edge cell (its ghost values depend on the state)
+    runs through the boundary hook and gives exactly the built-in copy face (type 0)."""
     prob, _, _, _ = load_run("solid96_rk2")
     vals = prob.in_values()
     vals["sdcopy_0"] = 1
-    with pytest.raises(RuntimeError, match="depends on the state"):
-        run_from_file(str(tmp_path), prob, extra_args=("--quiet",), in_values=vals)
+    hooked, builtin = tmp_path / "hooked", tmp_path / "builtin"
+    hooked.mkdir()
+    builtin.mkdir()
+    run_from_file(str(hooked), prob, extra_args=("--quiet",), in_values=vals)
+    plain = prob.with_(self_faces=(), bc=(0,) + tuple(prob.bc[1:]))
+    run_from_file(str(builtin), plain, extra_args=("--quiet",), in_values=plain.in_values())
+    last = sorted(f for f in os.listdir(hooked) if f.startswith("cons_"))[-1]
+    a, b = GaukukOutput(str(hooked / last)), GaukukOutput(str(builtin / last))
+    assert a.t == b.t
+    assert np.array_equal(a.active(), b.active())
 
 
 @pytest.mark.parametrize("name", ["kh64_rk3", "blast3d_z0_rk2"])

@@ -25,7 +25,10 @@ from tests._fixtures import GOLDEN
 pytestmark = pytest.mark.gpu
 
 SETUPS = ["blast", "disk", "disk_planet", "kh", "rt", "shock_tube", "solid_body", "wave_test", "blast3d",
-          "shock_tube3d"]
+          "shock_tube3d",
+          # this project's tests/golden/setup_statebd.cpp in the same plug-in format: self-defined
+          # faces whose ghost values depend on the state on all four sides (the boundary hook)
+          "statebd"]
 
 
 def rel_l1(a, b):
