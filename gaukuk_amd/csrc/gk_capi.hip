@@ -395,8 +395,10 @@ static void enqueue_steps(gk_ctx* c, int n) {
 // allows plus that test, and at most 256 per host round trip.  Steps past the end cost one early
 // return per launch.
 static int advance_chunk(const DevClock& h) {
+    unsigned long long mb = 0;
+    for (int s = 0; s < kCmaxSlots; s++) mb = std::max(mb, h.cmax_bits[h.parity][s]);
     double cm;
-    std::memcpy(&cm, &h.cmax_bits[h.parity], sizeof cm);
+    std::memcpy(&cm, &mb, sizeof cm);
     const double dt = h.cfl * h.drmin / cm;
     const double rem = h.t_next - h.t;
     double est = (dt > 0 && std::isfinite(dt)) ? std::ceil(std::max(rem, 0.0) / dt) + 1 : 256;
@@ -423,7 +425,7 @@ static gk_status refresh_cmax(gk_ctx* c) {
     unsigned long long one = 0;
     double floor_c = 1e-16;
     std::memcpy(&one, &floor_c, sizeof one);
-    c->hclock->cmax_bits[c->hclock->parity] = one;
+    for (int k = 0; k < kCmaxSlots; k++) c->hclock->cmax_bits[c->hclock->parity][k] = one;
     if ((s = write_clock(c)) != GK_OK) return s;
     const int cur = c->hclock->cur;
     const void* u = c->buf[cur ? 1 : 0];
@@ -783,8 +785,10 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
     std::memset(&h, 0, sizeof h);
     h.t = 0; h.dt = 1e10; h.t_next = 0; h.dt_until_output = 1e10;
     double floor_c = 1e-16;
-    std::memcpy(&h.cmax_bits[0], &floor_c, 8);
-    std::memcpy(&h.cmax_bits[1], &floor_c, 8);
+    for (int k = 0; k < kCmaxSlots; k++) {
+        std::memcpy(&h.cmax_bits[0][k], &floor_c, 8);
+        std::memcpy(&h.cmax_bits[1][k], &floor_c, 8);
+    }
     h.step = 0; h.stepmax = phys->stepmax; h.is_continue = 1; h.active = 0; h.parity = 0; h.cur = 0;
     h.cfl = phys->cfl; h.drmin = c->drmin;
     *c->hclock = h;

@@ -19,12 +19,18 @@ namespace gk {
 template <class R> using EosPolicy = GK_EOS<R>;
 using RiemannPolicy = GK_FLUX;
 
+// The running max of the wave speed is spread over this many words per parity (each wave of a
+// stage adds its max to one, chosen by its index): all waves of a launch can finish at once, and
+// device-scope atomics on a single word from thousands of waves serialize.  The step's dt takes the
+// max of the words, which equals the max over all waves whatever their order.
+constexpr int kCmaxSlots = 32;
+
 // Device-resident time state, so that whole steps (dt included) run without a host round
 // trip and can be captured in a HIP graph.  Mirrors the scalars of Sim::Advance
 // (src/sim.cpp:69-117): t, dt, dtUntilOutput, tNext, step, stepMax, isContinue.
 struct DevClock {
     double t, dt, t_next, dt_until_output;
-    unsigned long long cmax_bits[2];  // running max of |v|+c_s as IEEE bits, indexed by parity
+    unsigned long long cmax_bits[2][kCmaxSlots];  // running max of |v|+c_s as IEEE bits, by parity
     int step, stepmax, is_continue, active;
     int parity;      // which cmax slot the current step reads
     int cur;         // forward Euler: 1 when the current state lives in the second buffer

@@ -224,8 +224,10 @@ gk_status gk_ens_create(int count, const gk_grid* grids, const gk_phys* phys, gk
         DevClock& h = e->hclocks[p];
         std::memset(&h, 0, sizeof h);
         h.dt = 1e10; h.dt_until_output = 1e10;
-        std::memcpy(&h.cmax_bits[0], &floor_c, 8);
-        std::memcpy(&h.cmax_bits[1], &floor_c, 8);
+        for (int k = 0; k < kCmaxSlots; k++) {
+            std::memcpy(&h.cmax_bits[0][k], &floor_c, 8);
+            std::memcpy(&h.cmax_bits[1][k], &floor_c, 8);
+        }
         h.stepmax = phys[p].stepmax; h.is_continue = 1;
         h.cfl = phys[p].cfl;
         const double dx = (grids[p].xmax - grids[p].xmin) / grids[p].nx;
@@ -272,7 +274,7 @@ gk_status gk_ens_upload(gk_ens* e, int p, const double* host) {
     DevClock& h = e->hclocks[p];
     h.cur = 0;
     double floor_c = 1e-16;
-    std::memcpy(&h.cmax_bits[h.parity], &floor_c, 8);
+    for (int k = 0; k < kCmaxSlots; k++) std::memcpy(&h.cmax_bits[h.parity][k], &floor_c, 8);
     GK_ENS_TRY(e, hipMemcpyAsync(e->clocks + p, &h, sizeof(DevClock), hipMemcpyHostToDevice, e->stream));
     const size_t n = (size_t)v->plane * 5;
     if (e->prec == 32) {

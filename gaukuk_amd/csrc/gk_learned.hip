@@ -851,7 +851,7 @@ __global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArg
         if (tid == 0) {
             double mm = swmax[0];
             for (int w = 1; w < kLWaves; w++) mm = smax(mm, swmax[w]);
-            atomicMax(&clk->cmax_bits[clk->parity ^ 1], dbits(mm));
+            atomicMax(&clk->cmax_bits[clk->parity ^ 1][blockIdx.x % kCmaxSlots], dbits(mm));
         }
     }
 }
