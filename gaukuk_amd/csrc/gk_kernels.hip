@@ -15,7 +15,8 @@
 // are computed once per face, so every cell is converted once and every face solved once
 // (plus the strip halo).  Loads of the next row are issued before the current row's flux math.
 // Waves never exchange data, so there is no barrier; x neighbours move between lanes (DPP shifts
-// in FP64, permutes in FP32), and only the 3D variant uses LDS, as per-wave private storage.
+// in FP64, permutes in FP32), and LDS serves only as per-wave private storage (the 2D variants'
+// ring of raw input rows, the 3D variant's ring of neighbour-plane rows).
 #include "gk_device.hpp"
 
 namespace gk {
@@ -76,7 +77,7 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 // scalar address arithmetic.
 template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR, bool PACE = true>
 __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane, int kp = 0,
-                                      R* zring = nullptr) {
+                                      R* zring = nullptr, R* oring = nullptr) {
     constexpr bool FIXED = VAR == 1;
     constexpr bool THREED = VAR == 2;
     constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
@@ -361,15 +362,32 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             for (int n = 0; n < 5; n++) *reinterpret_cast<R*>(row + n * Pb) = res[n];
         }
     };
+    // 2D: the raw stage-input rows this wave loads, at its own column, stay in a 4-row ring in the
+    // wave's slice of LDS until the update two rows later takes them as its combine operand.
+    // Re-reading them from memory missed L2 (the rows were loaded three row iterations, tens of
+    // megabytes of traffic, earlier), and cost up to one extra read of the state per stage.  Owner
+    // lanes load their own column, so the ring holds exactly the operand (inner-region values
+    // included, before the gravity kick).  3D keeps its LDS for the z-plane ring.
+    constexpr bool RING = VAR != 2;
+    auto ring_put = [&](int r, const R u[5]) {
+        if (RING) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) oring[((r & 3) * 5 + n) * kWave + lane] = u[n];
+        }
+    };
+    auto ring_get = [&](int r, int n) -> R { return oring[((r & 3) * 5 + n) * kWave + lane]; };
     // operands of the update of row r-2, issued before the flux math
     auto load_operands = [&](int r, R bs[5], R ut[5]) {
         const long long o = (long long)(r - 2) * pitch;
         if (THREE) {
 #pragma unroll
-            for (int n = 0; n < 5; n++) { bs[n] = at(base + o, n, off_own); ut[n] = at(in + o, n, off_own); }
+            for (int n = 0; n < 5; n++) {
+                bs[n] = at(base + o, n, off_own);
+                ut[n] = RING ? ring_get(r - 2, n) : at(in + o, n, off_own);
+            }
         } else {
 #pragma unroll
-            for (int n = 0; n < 5; n++) { bs[n] = at(in + o, n, off_own); ut[n] = R(0); }
+            for (int n = 0; n < 5; n++) { bs[n] = RING ? ring_get(r - 2, n) : at(in + o, n, off_own); ut[n] = R(0); }
         }
         if (inner_at(r - 2)) {
             inner_load(r - 2, bs);
@@ -391,12 +409,14 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         finish_row(j0 - 1, u, W1);
 #pragma unroll
         for (int n = 0; n < 5; n++) u[n] = pre[n];
+        ring_put(j0, u);
         load_row(j0 + 1, pre);
         finish_row(j0, u, W2);
         R wR[5];
         yslope(W0, W1, W2, wR, wLy);   // wLy: left state of face j0
 #pragma unroll
         for (int n = 0; n < 5; n++) u[n] = pre[n];
+        ring_put(j0 + 1, u);
         load_row(j0 + 2, pre);
         finish_row(j0 + 1, u, W0);     // W0 now holds row j0+1
         R wLn[5];
@@ -426,6 +446,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         R u[5], bs[5], ut[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) u[n] = pre[n];
+        ring_put(r, u);
         load_row(min(r + 1, rlast), pre);
         if (EARLY_OPS) {
 #pragma unroll
@@ -498,7 +519,8 @@ void stage_kernel(const StageArgs<R> a) {
         const int kp = w / per_plane, r = w - kp * per_plane;
         march<R, Lim, THREE, GRAV, LAST, VAR>(a, r % a.strips, r / a.strips, lane, kp, zring[wid]);
     } else {
-        march<R, Lim, THREE, GRAV, LAST, VAR>(a, w % a.strips, w / a.strips, lane);
+        __shared__ R oring[kBlock / kWave][4 * 5 * kWave];
+        march<R, Lim, THREE, GRAV, LAST, VAR>(a, w % a.strips, w / a.strips, lane, 0, nullptr, oring[wid]);
     }
 }
 
@@ -597,6 +619,8 @@ template <class R, class Lim, bool THREE, bool GRAV, bool LAST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GK_WAVES_PER_EU)))
 void ens_stage_kernel(const EnsArgs<R> e) {
     const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ R oring[kBlock / kWave][4 * 5 * kWave];
     int item = 0;
     if (lane == 0) item = atomicAdd(e.counter, 1);
     item = __builtin_amdgcn_readfirstlane(item);
@@ -625,7 +649,8 @@ void ens_stage_kernel(const EnsArgs<R> e) {
             a.dxRec = d.dxRec; a.dyRec = d.dyRec;
             a.clock = clk;
             const int local = item - d.item_base;
-            march<R, Lim, THREE, GRAV, LAST, 0, false>(a, local % d.strips, local / d.strips, lane);
+            march<R, Lim, THREE, GRAV, LAST, 0, false>(a, local % d.strips, local / d.strips, lane, 0, nullptr,
+                                                       oring[wid]);
         }
         item = next;
     }
