@@ -21,9 +21,10 @@ struct gk_ens {
     size_t arena_elems = 0;
     DevClock* clocks = nullptr;
     DevClock* hclocks = nullptr;  // pinned mirror
-    int* counters = nullptr;      // one work-queue head per stage of a step
+    int* counters = nullptr;      // work-queue heads: kEnsQueues per stage of a step
     hipStream_t stream = nullptr;
     int total_items = 0;
+    int xcd_begin[kEnsQueues + 1] = {0};
     int seg_rows = 0;
     int waves_per_cu = 12;
     int blocks = 0;
@@ -64,6 +65,8 @@ static void ens_plan(gk_ens* e) {
         base += d.items;
     }
     e->total_items = base;
+    // kEnsQueues contiguous, equal shares of the items, one per XCD queue
+    for (int x = 0; x <= kEnsQueues; x++) e->xcd_begin[x] = (int)((long long)base * x / kEnsQueues);
     e->blocks = kNumCUs * e->waves_per_cu / (kBlock / kWave);
     // the looped baseline runs each problem as a single grid would run alone: every stage kernel
     // with the automatic plan for that grid's size at that kernel's occupancy (stage_plan)
@@ -97,7 +100,8 @@ static void ens_launch_stage(gk_ens* e, size_t q) {
     a.arena = (R*)e->arena;
     a.probs = e->dprobs;
     a.clocks = e->clocks;
-    a.counter = e->counters + q;
+    a.counter = e->counters + kEnsQueues * q;
+    for (int x = 0; x <= kEnsQueues; x++) a.xcd_begin[x] = e->xcd_begin[x];
     a.nprob = e->P;
     a.total_items = e->total_items;
     a.in_buf = s.in; a.base_buf = s.base; a.out_buf = s.out;
@@ -115,7 +119,7 @@ static void ens_launch_stage(gk_ens* e, size_t q) {
 static void ens_enqueue_step(gk_ens* e) {
     const int nthreads = 256, nb = (e->P + nthreads - 1) / nthreads;
     hipLaunchKernelGGL(ens_step_begin_kernel, dim3(nb), dim3(nthreads), 0, e->stream, e->clocks, e->P, e->counters,
-                       (int)e->stages.size());
+                       (int)e->stages.size() * kEnsQueues);
     for (size_t q = 0; q < e->stages.size(); q++) {
         if (e->prec == 32) ens_launch_stage<float>(e, q);
         else ens_launch_stage<double>(e, q);
@@ -216,7 +220,7 @@ gk_status gk_ens_create(int count, const gk_grid* grids, const gk_phys* phys, gk
     if (hipMalloc(&e->arena, e->arena_elems * e->esz) != hipSuccess) return bail(GK_ENOMEM, "arena allocation failed");
     if (hipMalloc(&e->dprobs, sizeof(EnsProblem) * count) != hipSuccess ||
         hipMalloc(&e->clocks, sizeof(DevClock) * count) != hipSuccess ||
-        hipMalloc(&e->counters, sizeof(int) * 4) != hipSuccess ||
+        hipMalloc(&e->counters, sizeof(int) * 4 * kEnsQueues) != hipSuccess ||
         hipHostMalloc((void**)&e->hclocks, sizeof(DevClock) * count) != hipSuccess)
         return bail(GK_ENOMEM, "auxiliary allocation failed");
     double floor_c = 1e-16;
@@ -241,7 +245,7 @@ gk_status gk_ens_create(int count, const gk_grid* grids, const gk_phys* phys, gk
         v.uploaded = true;
     }
     if (hipMemcpy(e->clocks, e->hclocks, sizeof(DevClock) * count, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(e->counters, 0, sizeof(int) * 4) != hipSuccess)
+        hipMemset(e->counters, 0, sizeof(int) * 4 * kEnsQueues) != hipSuccess)
         return bail(GK_EHIP, "clock init failed");
     ens_plan(e);
     if (ens_upload_table(e) != GK_OK) return bail(GK_EHIP, "descriptor upload failed");

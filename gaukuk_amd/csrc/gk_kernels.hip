@@ -596,6 +596,12 @@ __global__ __launch_bounds__(256) void inner_reset_kernel(R* u, const R* saved, 
 // split into work items (strip, segment); a persistent grid of waves claims items from an atomic
 // counter until all are done, so large and small problems share the machine without a launch per
 // problem.  Each problem has its own clock (its own dt).
+// Work queues of the ensemble kernel: one per XCD (workgroups are dealt to the 8 XCDs round-robin,
+// so workgroup b runs on XCD b % 8).  Each queue holds a contiguous eighth of the items, so
+// neighbouring strips and segments of a problem -- which share halo rows and the cache lines at
+// strip edges -- run on the same XCD and meet in its L2.
+constexpr int kEnsQueues = 8;
+
 struct EnsProblem {
     long long off[3];     // element offsets of the problem's state buffers A, B, C in the arena
     long long plane;
@@ -609,8 +615,9 @@ struct EnsArgs {
     R* arena;
     const EnsProblem* probs;
     DevClock* clocks;
-    int* counter;         // work-queue head of this stage (reset by ens_step_begin_kernel)
+    int* counter;         // this stage's work-queue heads, one per XCD (reset by ens_step_begin_kernel)
     int nprob, total_items;
+    int xcd_begin[kEnsQueues + 1];   // queue x hands out items [xcd_begin[x], xcd_begin[x + 1])
     int in_buf, base_buf, out_buf;
     StageArgs<R> proto;   // stage constants (coefficients, flags, gravity, EOS)
 };
@@ -621,13 +628,16 @@ void ens_stage_kernel(const EnsArgs<R> e) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     __shared__ R oring[kBlock / kWave][4 * 5 * kWave];
+    const int q = blockIdx.x % kEnsQueues;
+    const int lo = e.xcd_begin[q], hi = e.xcd_begin[q + 1];
+    int* head = e.counter + q;
     int item = 0;
-    if (lane == 0) item = atomicAdd(e.counter, 1);
+    if (lane == 0) item = lo + atomicAdd(head, 1);
     item = __builtin_amdgcn_readfirstlane(item);
-    while (item < e.total_items) {
+    while (item < hi) {
         // claim the next item before working on this one, to hide the atomic's latency
         int next = 0;
-        if (lane == 0) next = atomicAdd(e.counter, 1);
+        if (lane == 0) next = lo + atomicAdd(head, 1);
         next = __builtin_amdgcn_readfirstlane(next);
         // the problem owning `item`: last p with item_base <= item (binary search, scalar loads)
         int lo = 0, hi = e.nprob - 1;
