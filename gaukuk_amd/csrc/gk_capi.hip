@@ -96,7 +96,7 @@ struct gk_ctx {
     // fixed (type 4) faces (gk_set_fixed_boundary): the saved state in the reference layout, and
     // on the device the two ghost layers nearest each face (xl, xr, yl, yr) in the state precision
     std::vector<double> fixed_ref;
-    void* fix[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* fix[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // inner fixed region: saved state (state layout) and per-row column intervals; bounding box
     void* inner = nullptr;
     int2* inner_rows = nullptr;
@@ -141,13 +141,17 @@ static gk_status fail(gk_ctx* c, gk_status s, const std::string& m) {
     return s;
 }
 
+// faces of the grid: x and y, and z in 3D
+static int nfaces(const gk_ctx* c) { return c->nz > 1 ? 6 : 4; }
 static bool has_fixed_face(const gk_ctx* c) {
-    return c->bc[0] == 4 || c->bc[1] == 4 || c->bc[2] == 4 || c->bc[3] == 4;
+    for (int f = 0; f < nfaces(c); f++)
+        if (c->bc[f] == 4) return true;
+    return false;
 }
 static bool hooked(const gk_ctx* c, int f) { return c->hook && ((c->hook_mask >> f) & 1); }
 // type-4 faces whose data must come from gk_set_fixed_boundary (not refreshed by a hook)
 static bool has_enrolled_face(const gk_ctx* c) {
-    for (int f = 0; f < 4; f++)
+    for (int f = 0; f < nfaces(c); f++)
         if (c->bc[f] == 4 && !hooked(c, f)) return true;
     return false;
 }
@@ -323,6 +327,7 @@ static StageArgs<R> make_args(gk_ctx* c, const StageSpec& s) {
     a.xc = c->xc; a.yc = c->yc; a.zc = c->zc; a.ng = c->grid.nghost;
     a.fxl = (const R*)c->fix[0]; a.fxr = (const R*)c->fix[1];
     a.fyl = (const R*)c->fix[2]; a.fyr = (const R*)c->fix[3];
+    a.fzl = (const R*)c->fix[4]; a.fzr = (const R*)c->fix[5];
     a.inner = (const R*)c->inner; a.inner_rows = c->inner_rows;
     return a;
 }
@@ -343,8 +348,11 @@ static void launch_stage_f(gk_ctx* c, const StageArgs<R>& a, bool last) {
 
 template <class R, class L, bool THREE>
 static void launch_stage_t(gk_ctx* c, const StageArgs<R>& a, bool last) {
-    if (c->nz > 1) launch_stage_f<R, L, THREE, 2>(c, a, last);
-    else if (has_fixed_face(c) || c->inner) launch_stage_f<R, L, THREE, 1>(c, a, last);
+    const bool saved = has_fixed_face(c) || c->inner;
+    if (c->nz > 1) {
+        if (saved) launch_stage_f<R, L, THREE, 3>(c, a, last);
+        else launch_stage_f<R, L, THREE, 2>(c, a, last);
+    } else if (saved) launch_stage_f<R, L, THREE, 1>(c, a, last);
     else launch_stage_f<R, L, THREE, 0>(c, a, last);
 }
 
@@ -468,10 +476,10 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
                 for (int i = 0; i < nx; i++) at(n, k + g, j + g, i + g) = (R)act[(((size_t)n * nz + k) * ny + j) * nx + i];
     // ghost fill over the active tangential range, x faces, y faces, then z faces
     // (src/boundary/boundary.cpp:127-142); fixed faces copy the saved state at the same position
-    // (src/boundary/fixed.cpp:16-123); type-0 z faces leave their planes zero (the reference's z copy
+    // (src/boundary/fixed.cpp:16-179); type-0 z faces leave their planes zero (the reference's z copy
     // refills the y ghosts instead, boundary.cpp:90-122)
-    auto saved = [&](int n, int j, int i) {
-        return (R)c->fixed_ref[(((size_t)n * (1 + 2 * g) + g) * ly + j) * lx + i];
+    auto saved = [&](int n, int k, int j, int i) {
+        return (R)c->fixed_ref[(((size_t)n * lz + k) * ly + j) * lx + i];
     };
     for (int k = g; k < g + nz; k++) {
         for (int j = g; j < g + ny; j++)
@@ -481,7 +489,7 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
                 int kind;
                 const int si = bc_remap(i - g, nx, c->bc[0], c->bc[1], kind) + g;
                 R u[5];
-                for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, k, j, si);
+                for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, k, j, i) : at(n, k, j, si);
                 bc_transform(u, 1, kind);
                 for (int n = 0; n < 5; n++) at(n, k, j, i) = u[n];
             }
@@ -493,7 +501,7 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
                     int kind;
                     const int sj = bc_remap(j - g, ny, c->bc[2], c->bc[3], kind) + g;
                     R u[5];
-                    for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, j, i) : at(n, k, sj, i);
+                    for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, k, j, i) : at(n, k, sj, i);
                     bc_transform(u, 2, kind);
                     for (int n = 0; n < 5; n++) at(n, k, j, i) = u[n];
                 }
@@ -503,13 +511,13 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
         for (int k = 0; k < lz; k++) {
             if (k >= g && k < g + nz) continue;
             const int zt = k < g ? c->bc[4] : c->bc[5];
-            if (zt < 1 || zt > 3) continue;
+            if (zt < 1 || zt > 4) continue;
             int kind;
             const int sk = bc_remap(k - g, nz, c->bc[4], c->bc[5], kind) + g;
             for (int j = g; j < g + ny; j++)
                 for (int i = g; i < g + nx; i++) {
                     R u[5];
-                    for (int n = 0; n < 5; n++) u[n] = at(n, sk, j, i);
+                    for (int n = 0; n < 5; n++) u[n] = kind == 4 ? saved(n, k, j, i) : at(n, sk, j, i);
                     bc_transform(u, 3, kind);
                     for (int n = 0; n < 5; n++) at(n, k, j, i) = u[n];
                 }
@@ -670,40 +678,50 @@ static gk_status upload_real(gk_ctx* c, void** dst, const std::vector<double>& v
 }
 
 // Boundary::FixedBDInner's region (src/boundary/fixed.cpp:181-207): the central half-box of the
-// active cells, restricted to cell centres within 0.5 of the origin; per active row the region is
-// one run of columns (x^2 grows with |x| and rounding keeps that order)
+// active cells in x and y, on every active plane, restricted to cell centres within 0.5 of the
+// origin; per active row the region is one run of columns (x^2 grows with |x| and rounding keeps
+// that order).  Rows are numbered k * ny + j over the planes, which is also their order in the
+// state (planes of ny rows follow each other), so the bounding rows [inner_j0, inner_j1) span
+// planes in 3D.
 static gk_status set_inner(gk_ctx* c) {
-    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, nz = c->nz;
     const int ib = g, ie = g + nx, jb = g, je = g + ny;
     const int il = static_cast<int>(ib + 0.25 * (ie - ib)), ir = static_cast<int>(ib + 0.75 * (ie - ib));
     const int jl = static_cast<int>(jb + 0.25 * (je - jb)), jr = static_cast<int>(jb + 0.75 * (je - jb));
     std::vector<double> xc, yc;
     double zc;
     domain_centres(c, xc, yc, zc);
-    std::vector<int2> rows((size_t)ny, make_int2(0, 0));
-    c->inner_j0 = ny; c->inner_j1 = 0; c->inner_i0 = nx; c->inner_i1 = 0;
-    for (int j = jl; j < jr; j++) {
-        int lo = -1, hi = -1;
-        for (int i = il; i < ir; i++) {
-            const double r2 = xc[i] * xc[i] + yc[j] * yc[j] + zc * zc;
-            if (r2 < 0.5 * 0.5) {
-                if (lo < 0) lo = i;
-                else if (hi != i) return fail(c, GK_EINVAL, "inner fixed region is not one run of columns per row");
-                hi = i + 1;
+    const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
+    std::vector<int2> rows((size_t)ny * nz, make_int2(0, 0));
+    c->inner_j0 = ny * nz; c->inner_j1 = 0; c->inner_i0 = nx; c->inner_i1 = 0;
+    for (int k = 0; k < nz; k++) {
+        // Domain::zc of active plane k (src/sim.cpp:39); the single plane's centre in 2D
+        const double z = nz > 1 ? c->grid.zmin - g * dz + 0.5 * dz + (k + g) * dz : zc;
+        for (int j = jl; j < jr; j++) {
+            int lo = -1, hi = -1;
+            for (int i = il; i < ir; i++) {
+                const double r2 = xc[i] * xc[i] + yc[j] * yc[j] + z * z;
+                if (r2 < 0.5 * 0.5) {
+                    if (lo < 0) lo = i;
+                    else if (hi != i) return fail(c, GK_EINVAL, "inner fixed region is not one run of columns per row");
+                    hi = i + 1;
+                }
             }
+            if (lo < 0) continue;
+            const int row = k * ny + j - g;
+            rows[row] = make_int2(lo - g, hi - g);
+            c->inner_j0 = std::min(c->inner_j0, row); c->inner_j1 = std::max(c->inner_j1, row + 1);
+            c->inner_i0 = std::min(c->inner_i0, lo - g); c->inner_i1 = std::max(c->inner_i1, hi - g);
         }
-        if (lo < 0) continue;
-        rows[j - g] = make_int2(lo - g, hi - g);
-        c->inner_j0 = std::min(c->inner_j0, j - g); c->inner_j1 = std::max(c->inner_j1, j - g + 1);
-        c->inner_i0 = std::min(c->inner_i0, lo - g); c->inner_i1 = std::max(c->inner_i1, hi - g);
     }
     std::vector<double> saved((size_t)c->plane * 5);
-    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = 1 + 2 * g;
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = nz + 2 * g;
     for (int n = 0; n < 5; n++)
-        for (int j = 0; j < ny; j++)
-            for (int i = 0; i < nx; i++)
-                saved[(size_t)n * c->plane + (size_t)j * c->pitch + i] =
-                    c->fixed_ref[(((size_t)n * lz + g) * ly + j + g) * lx + i + g];
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++)
+                    saved[(size_t)n * c->plane + (size_t)k * c->plane2 + (size_t)j * c->pitch + i] =
+                        c->fixed_ref[(((size_t)n * lz + k + g) * ly + j + g) * lx + i + g];
     if (gk_status s = upload_vec(c, &c->inner_rows, rows)) return s;
     return upload_real(c, &c->inner, saved);
 }
@@ -719,10 +737,7 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
         g_create_error = "Invalid grid parameters from config"; return GK_EINVAL;
     }
     if (grid->nz > 1) {
-        // 3D: face types 0-3 on all six faces (fixed faces, the inner region, immersed shells,
-        // ensembles and the learned correction are 2D only)
-        for (int f = 0; f < 6; f++)
-            if (phys->bc[f] == 4) { g_create_error = "3D grids take face types 0-3"; return GK_EINVAL; }
+        // 3D: immersed shells, boundary hooks, ensembles and the learned correction are 2D only
         if (grid->ny < 2) { g_create_error = "3D grids need NY >= 2"; return GK_EINVAL; }
     }
     if (grid->nghost < 2) { g_create_error = "NGhost must be >= 2 for the PLM stencil"; return GK_EINVAL; }
@@ -815,7 +830,7 @@ void gk_destroy(gk_ctx* c) {
     if (c->xc) (void)hipFree(c->xc);
     if (c->yc) (void)hipFree(c->yc);
     if (c->lstats) (void)hipFree(c->lstats);
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 6; k++)
         if (c->fix[k]) (void)hipFree(c->fix[k]);
     if (c->inner) (void)hipFree(c->inner);
     if (c->inner_rows) (void)hipFree(c->inner_rows);
@@ -1019,32 +1034,42 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
 gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host, int fix_inner) {
     if (!c || !host) return GK_EINVAL;
     if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
-    if (c->nz > 1) return fail(c, GK_EINVAL, "fixed faces and the inner region are 2D only");
-    // Boundary::EnrollFixedBoundaryData: the ghost layers of the given state become the fixed faces'
-    // values; the stencil reads the two layers nearest each face
-    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
-    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = 1 + 2 * g;
+    // Boundary::EnrollFixedBoundaryData (src/boundary/fixed.cpp:242-269): the ghost layers of the
+    // given state become the fixed faces' values; the stencil reads the two layers nearest each x
+    // and y face on every active plane, and the nearest plane of each z face
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, nz = c->nz;
+    const int lx = nx + 2 * g, ly = ny + 2 * g, lz = nz + 2 * g;
     const size_t total = 5 * (size_t)lx * ly * lz;
     c->fixed_ref.resize(total);
     for (size_t k = 0; k < total; k++)
         c->fixed_ref[k] = c->prec == 32 ? (double)((const float*)host)[k] : ((const double*)host)[k];
-    auto ref = [&](int n, int j, int i) { return c->fixed_ref[(((size_t)n * lz + g) * ly + j) * lx + i]; };
-    std::vector<double> face[4];
-    face[0].resize(10 * (size_t)ny); face[1].resize(10 * (size_t)ny);
-    face[2].resize(10 * (size_t)nx); face[3].resize(10 * (size_t)nx);
+    auto ref = [&](int n, int k, int j, int i) { return c->fixed_ref[(((size_t)n * lz + k) * ly + j) * lx + i]; };
+    const int nf = nfaces(c);
+    std::vector<double> face[6];
+    face[0].resize(10 * (size_t)ny * nz); face[1].resize(10 * (size_t)ny * nz);
+    face[2].resize(10 * (size_t)nx * nz); face[3].resize(10 * (size_t)nx * nz);
+    if (nf == 6) { face[4].resize(5 * (size_t)nx * ny); face[5].resize(5 * (size_t)nx * ny); }
     for (int n = 0; n < 5; n++)
-        for (int d = 0; d < 2; d++) {
-            for (int j = 0; j < ny; j++) {
-                face[0][((size_t)n * ny + j) * 2 + d] = ref(n, j + g, g - 1 - d);
-                face[1][((size_t)n * ny + j) * 2 + d] = ref(n, j + g, g + nx + d);
+        for (int k = 0; k < nz; k++)
+            for (int d = 0; d < 2; d++) {
+                for (int j = 0; j < ny; j++) {
+                    face[0][(((size_t)n * nz + k) * ny + j) * 2 + d] = ref(n, k + g, j + g, g - 1 - d);
+                    face[1][(((size_t)n * nz + k) * ny + j) * 2 + d] = ref(n, k + g, j + g, g + nx + d);
+                }
+                for (int i = 0; i < nx; i++) {
+                    face[2][(((size_t)n * nz + k) * 2 + d) * nx + i] = ref(n, k + g, g - 1 - d, i + g);
+                    face[3][(((size_t)n * nz + k) * 2 + d) * nx + i] = ref(n, k + g, g + ny + d, i + g);
+                }
             }
-            for (int i = 0; i < nx; i++) {
-                face[2][((size_t)n * 2 + d) * nx + i] = ref(n, g - 1 - d, i + g);
-                face[3][((size_t)n * 2 + d) * nx + i] = ref(n, g + ny + d, i + g);
-            }
-        }
+    if (nf == 6)
+        for (int n = 0; n < 5; n++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    face[4][((size_t)n * ny + j) * nx + i] = ref(n, g - 1, j + g, i + g);
+                    face[5][((size_t)n * ny + j) * nx + i] = ref(n, g + nz, j + g, i + g);
+                }
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    for (int f = 0; f < 4; f++) {
+    for (int f = 0; f < nf; f++) {
         const size_t cnt = face[f].size();
         if (!c->fix[f]) GK_TRY_HIP(c, hipMalloc(&c->fix[f], cnt * c->esz));
         if (c->prec == 32) {

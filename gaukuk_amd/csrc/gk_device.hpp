@@ -68,14 +68,18 @@ struct StageArgs {
     int npoint;
     double pm0[5], pm1[5];       // gm, x, y, z, rs of each mass
     double ab, omega, theta1;    // the binary's separation, angular rate and initial phase
-    // fixed (type 4) faces: the two saved ghost layers nearest each face, by distance d from it:
-    // x faces [5][ny][2] (element (n, j, d)), y faces [5][2][nx] (element (n, d, i))
+    // fixed (type 4) faces: the two saved ghost layers nearest each face, by distance d from it,
+    // per z plane k: x faces [5][nz][ny][2] (element (n, k, j, d)), y faces [5][nz][2][nx]
+    // (element (n, k, d, i)) (nz = 1 in 2D); 3D z faces the nearest saved plane [5][ny][nx]
     const R* fxl;
     const R* fxr;
     const R* fyl;
     const R* fyr;
+    const R* fzl;
+    const R* fzr;
     // inner fixed region (EnrollFixedBoundaryData with fixInner): the saved state (state layout)
-    // and per active row the column interval [x, y) of region cells (empty when x >= y)
+    // and per active row (k * ny + j) the column interval [x, y) of region cells (empty when
+    // x >= y)
     const R* inner;
     const int2* inner_rows;
     // 3D (nz > 1): planes of ny rows follow each other in every state plane of `plane` elements.

@@ -71,15 +71,16 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 // GRAV: constant gravity is enrolled; LAST: the stage ends the step (second half-kick, next
 // step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.  VAR selects
 // the boundary/geometry variant, so that plain 2D grids run code with no trace of the others:
-// 0 plain 2D; 1 2D with fixed faces or the inner fixed region (saved data); 2 3D (nz > 1).
+// 0 plain 2D; 1 2D with fixed faces or the inner fixed region (saved data); 2 3D (nz > 1); 3 3D
+// with saved data.
 // One wave advances the cells of strip `strip`, segment `seg` of one grid by one stage.
 // `strip` and `seg` are wave-uniform (scalar registers), so row logic is scalar branches and
 // scalar address arithmetic.
 template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR, bool PACE = true>
 __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane, int kp = 0,
                                       R* zring = nullptr, R* oring = nullptr) {
-    constexpr bool FIXED = VAR == 1;
-    constexpr bool THREED = VAR == 2;
+    constexpr bool FIXED = VAR == 1 || VAR == 3;
+    constexpr bool THREED = VAR >= 2;
     constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
     constexpr bool KICK_BASE = GRAV && THREE; // the combine base is U^n
     DevClock* clk = a.clock;
@@ -150,29 +151,36 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     };
     // fixed (type 4) x faces: only the edge strips hold their ghost lanes
     const bool xfixed = FIXED && xedge && (a.bc[0] == 4 || a.bc[1] == 4);
+    const int nzf = THREED ? a.nz : 1;   // planes of the saved x and y face layers
+    // a ghost row of a fixed y face in plane kz: the saved layer rs (distance from the face)
+    auto fixed_yrow = [&](int kz, int r, int rs, R u[5]) {
+        const R* F = r < 0 ? a.fyl : a.fyr;
+        const long long pl = 2LL * a.nx * nzf, o = ((long long)kz * 2 + rs) * a.nx + cs;
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = F[n * pl + o];
+    };
     // inner fixed region (src/boundary/fixed.cpp:181-207): every read of a stage input or of U^n at
     // a region cell returns the saved state, unkicked (the reference resets those cells after the
-    // face fills and after the first half-kick); results are stored as computed
-    auto inner_at = [&](int r) -> bool {
+    // face fills and after the first half-kick); results are stored as computed.  Region rows are
+    // indexed kz * ny + r (one plane in 2D).
+    auto inner_at_k = [&](int kz, int r) -> bool {
         if (!FIXED || !a.inner || r < 0 || r >= a.ny) return false;
-        const int2 iv = a.inner_rows[r];
+        const int2 iv = a.inner_rows[kz * a.ny + r];
         return c >= iv.x && c < iv.y;
     };
-    auto inner_load = [&](int r, R u[5]) {
-        const long long o = (long long)r * pitch + c;
+    auto inner_load_k = [&](int kz, int r, R u[5]) {
+        const long long o = ((long long)kz * a.ny + r) * pitch + c;
 #pragma unroll
         for (int n = 0; n < 5; n++) u[n] = a.inner[n * P + o];
     };
+    auto inner_at = [&](int r) -> bool { return inner_at_k(kp, r); };
+    auto inner_load = [&](int r, R u[5]) { inner_load_k(kp, r, u); };
     // raw conserved values of row r (ghost rows remapped; the row index is wave-uniform)
     auto load_row = [&](int r, R u[5]) {
         int ky;
         const int rs = bc_remap(r, a.ny, a.bc[2], a.bc[3], ky);
         if (FIXED && ky == 4) {
-            // a ghost row of a fixed y face: the saved layer rs of that face
-            const R* F = r < 0 ? a.fyl : a.fyr;
-            const long long pl = 2LL * a.nx, o = (long long)rs * a.nx + cs;
-#pragma unroll
-            for (int n = 0; n < 5; n++) u[n] = F[n * pl + o];
+            fixed_yrow(kp, r, rs, u);
         } else {
             const R* row = in + (long long)rs * pitch;
 #pragma unroll
@@ -182,7 +190,8 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             // a ghost column of a fixed x face (cs is its distance from the face); corner cells
             // are never read, so ghost rows take the nearest active row
             const R* F = c < 0 ? a.fxl : a.fxr;
-            const long long pl = 2LL * a.ny, o = (long long)min(max(r, 0), a.ny - 1) * 2 + cs;
+            const long long pl = 2LL * a.ny * nzf;
+            const long long o = ((long long)kp * a.ny + min(max(r, 0), a.ny - 1)) * 2 + cs;
 #pragma unroll
             for (int n = 0; n < 5; n++) u[n] = F[n * pl + o];
         }
@@ -257,6 +266,15 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         if (kz >= 0 && kz < a.nz) {
             int ky;
             const int rs = bc_remap(jz, a.ny, a.bc[2], a.bc[3], ky);
+            // saved data (fixed y-face rows, inner-region cells) is read as it is: no kick
+            if (FIXED && ky == 4) {
+                fixed_yrow(kz, jz, rs, u);
+                return;
+            }
+            if (inner_at_k(kz, jz)) {
+                inner_load_k(kz, jz, u);
+                return;
+            }
             const R* row = in3 + kz * a.plane2 + (long long)rs * pitch;
 #pragma unroll
             for (int n = 0; n < 5; n++) u[n] = at(row, n, off_in);
@@ -265,10 +283,17 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
             return;
         }
         // a z ghost plane: z faces of types 1-3 fill its active rows (periodic.cpp:135-190,
-        // reflective.cpp:170-230, outflow.cpp:216-300); type 0 registers the y copy instead
-        // (boundary.cpp:90-122), and no face fills edge cells
+        // reflective.cpp:170-230, outflow.cpp:216-300) from the active planes as they stand before
+        // the inner region's reset (boundary.cpp:127-142); a fixed face restores its saved plane
+        // (fixed.cpp:126-179); type 0 registers the y copy instead (boundary.cpp:90-122), and no
+        // face fills edge cells
         const int zt = kz < 0 ? a.bc[4] : a.bc[5];
-        if ((zt == 1 || zt == 2 || zt == 3) && jz >= 0 && jz < a.ny) {
+        if (FIXED && zt == 4 && jz >= 0 && jz < a.ny) {
+            const R* F = kz < 0 ? a.fzl : a.fzr;
+            const long long pl = (long long)a.ny * a.nx, o = (long long)jz * a.nx + cown;
+#pragma unroll
+            for (int n = 0; n < 5; n++) u[n] = F[n * pl + o];
+        } else if ((zt == 1 || zt == 2 || zt == 3) && jz >= 0 && jz < a.ny) {
             int kk;
             const int ks = bc_remap(kz, a.nz, a.bc[4], a.bc[5], kk);
             const R* row = in3 + ks * a.plane2 + (long long)jz * pitch;
@@ -368,7 +393,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     // megabytes of traffic, earlier), and cost up to one extra read of the state per stage.  Owner
     // lanes load their own column, so the ring holds exactly the operand (inner-region values
     // included, before the gravity kick).  3D keeps its LDS for the z-plane ring.
-    constexpr bool RING = VAR != 2;
+    constexpr bool RING = !THREED;
     auto ring_put = [&](int r, const R u[5]) {
         if (RING) {
 #pragma unroll
@@ -504,14 +529,14 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 // w / strips.
 // 3D waves hold the z-face work on top of the 2D march: two waves per SIMD keep it out of scratch.
 template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VAR == 2 ? 2 : GK_WAVES_PER_EU)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VAR >= 2 ? 2 : GK_WAVES_PER_EU)))
 void stage_kernel(const StageArgs<R> a) {
     if (!a.clock->active) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
     if (w >= a.nwaves) return;
-    if constexpr (VAR == 2) {
+    if constexpr (VAR >= 2) {
         // 3D: strip fastest, then segment, then plane (waves of neighbouring planes share an XCD);
         // each wave keeps its neighbour-plane row ring in its own LDS slice
         __shared__ R zring[kBlock / kWave][2 * 3 * 5 * kWave];

@@ -226,14 +226,23 @@ class Solver:
         enrolled = prob.enrolled_state()
         if enrolled is not None:
             full = np.zeros(self._ref_shape())
-            full[:, prob.ng] = enrolled
+            if prob.nz > 1:
+                full[:] = enrolled
+            else:
+                full[:, prob.ng] = enrolled
             self.set_fixed_boundary(full, prob.fix_inner)
+            self._enrolled_ref = full
         if prob.shell:
             self._check(self.L.gk_set_immersed_shell(self.h, *[float(v) for v in prob.shell]))
         if seg_rows:
             self._check(self.L.gk_set_tile(self.h, seg_rows))
         if upload and prob.cons is not None:
-            self.upload(prob.cons)
+            if prob.nz > 1 and enrolled is not None:
+                # the setup's array holds the enrolled ghost values too, and in 3D the cells of the
+                # nearest z ghost planes that no face fills keep them
+                self.upload_ref_layout(self._enrolled_ref)
+            else:
+                self.upload(prob.cons)
 
     def _check(self, st):
         if st != 0:
@@ -258,6 +267,15 @@ class Solver:
         a = np.ascontiguousarray(cons, np.float64)
         assert a.shape == self._active_shape()
         self._check(self.L.gk_upload_active(self.h, _dp(a)))
+        self._bd_done = False
+
+    def upload_ref_layout(self, ref: np.ndarray):
+        """gk_upload_cons: a whole reference-layout state [5][NZ+2g][NY+2g][NX+2g] (ghost cells
+        included) in the solver's precision."""
+        dt = np.float64 if self.precision == 64 else np.float32
+        a = np.ascontiguousarray(ref, dt)
+        assert a.shape == self._ref_shape()
+        self._check(self.L.gk_upload_cons(self.h, a.ctypes.data))
         self._bd_done = False
 
     def _begin(self):

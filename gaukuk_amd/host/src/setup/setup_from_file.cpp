@@ -2,7 +2,8 @@
 // $GK_IC_FILE: int32 nvar (5), int32 nx, int32 ny, float64 cons[5][nz][ny][nx]; config keys
 // gx, gy, gz enroll constant gravity; npoint and pg<q>_* enroll point masses; fixedbd = 1 fills
 // every ghost cell with the conserved state fix_0..fix_4, fixedbd = 2 with the plane read from
-// $GK_FIXED_FILE, and enrolls it as fixed-face data (fixinner = 1: with the inner region);
+// $GK_FIXED_FILE, fixedbd = 3 with the whole ghost-inclusive 3D array read from it, and enrolls it
+// as fixed-face data (fixinner = 1: with the inner region);
 // selfbd_<f> = 1 enrolls a self-defined face f (0 x left .. 3 y right) that writes the conserved
 // state sd<f>_0..sd<f>_4 into its ghost cells (with sdcopy_<f> = 1 it copies the edge cell
 // instead, which depends on the state); immersed = 1 sets up the immersed shell
@@ -42,6 +43,27 @@ static void ReadFixedPlane(TArray<Real>& cons, const Grid& grid) {
             for (int j = 0; j < grid.leny; j++)
                 for (int i = 0; i < grid.lenx; i++)
                     cons(v, k, j, i) = (Real)buf[((size_t)v * grid.leny + j) * grid.lenx + i];
+}
+
+// fixedbd = 3: the ghost-inclusive array [5][NZ+2g][NY+2g][NX+2g] from $GK_FIXED_FILE (int32 nvar,
+// lenx, leny, lenz; float64 data)
+static void ReadFixedVolume(TArray<Real>& cons, const Grid& grid) {
+    const char* path = std::getenv("GK_FIXED_FILE");
+    if (!path) throw std::runtime_error("fixedbd = 3 needs GK_FIXED_FILE");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    int hdr[4] = {0, 0, 0, 0};
+    const bool ok = std::fread(hdr, sizeof(int), 4, f) == 4 && hdr[0] == NVar && hdr[1] == grid.lenx &&
+                    hdr[2] == grid.leny && hdr[3] == grid.lenz;
+    std::vector<double> buf;
+    if (ok) {
+        buf.resize(cons.GetSize());
+        if (std::fread(buf.data(), sizeof(double), buf.size(), f) != buf.size()) buf.clear();
+    }
+    std::fclose(f);
+    if (!ok) throw std::runtime_error("fixed-data file header does not match the ghost-inclusive grid");
+    if (buf.empty()) throw std::runtime_error("short fixed data");
+    for (size_t q = 0; q < buf.size(); q++) cons.data()[q] = (Real)buf[q];
 }
 
 // a self-defined face writing one conserved state into its ghost cells (active tangential range)
@@ -88,6 +110,7 @@ void Sim::Setup() {
     Config& c = Config::getInstance();
     const int fixedbd = (int)c.get("fixedbd", 0.0);
     if (fixedbd == 2) ReadFixedPlane(cons, grid);
+    if (fixedbd == 3) ReadFixedVolume(cons, grid);
     if (fixedbd == 1)
         for (int v = 0; v < NVar; v++) {
             const Real fv = (Real)c.get("fix_" + std::to_string(v), 0.0);
@@ -102,7 +125,7 @@ void Sim::Setup() {
                     cons(v, grid.kb + k, grid.jb + j, grid.ib + i) =
                         (Real)buf[(((size_t)v * grid.nz + k) * grid.ny + j) * grid.nx + i];
     const bool fixinner = (int)c.get("fixinner", 0.0) == 1;
-    if (fixedbd == 1 || fixedbd == 2 || fixinner) boundary.EnrollFixedBoundaryData(cons, grid, fixinner);
+    if ((fixedbd >= 1 && fixedbd <= 3) || fixinner) boundary.EnrollFixedBoundaryData(cons, grid, fixinner);
     if ((int)c.get("selfbd_0", 0.0) == 1) boundary.EnrollSelfDefineBDXL();
     if ((int)c.get("selfbd_1", 0.0) == 1) boundary.EnrollSelfDefineBDXR();
     if ((int)c.get("selfbd_2", 0.0) == 1) boundary.EnrollSelfDefineBDYL();

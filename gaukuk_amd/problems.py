@@ -59,7 +59,8 @@ class Problem:
     point: tuple = ()
     # fixed (type 4) boundaries: what the setup's ghost cells hold when it enrolls the fixed data
     # (Boundary::EnrollFixedBoundaryData, src/boundary/fixed.cpp:242-269): one conserved state for
-    # every ghost cell (5 values) or a ghost-inclusive array [5][NY+2g][NX+2g]
+    # every ghost cell (5 values) or a ghost-inclusive array [5][NY+2g][NX+2g] ([5][NZ+2g][NY+2g][NX+2g]
+    # in 3D)
     fixed: tuple = ()
     # self-defined faces (Boundary::EnrollSelfDefineBD*, src/boundary/boundary.hpp:28-45) of the
     # state-independent kind: (face, conserved state) pairs, face 0..3 = xleft, xright, yleft,
@@ -77,14 +78,20 @@ class Problem:
     def with_(self, **kw) -> "Problem":
         return replace(self, **kw)
 
+    def ghost_shape(self) -> tuple:
+        """The reference-layout state shape: [5][NY+2g][NX+2g] in 2D, [5][NZ+2g][NY+2g][NX+2g] in 3D."""
+        g = self.ng
+        plane = (self.ny + 2 * g, self.nx + 2 * g)
+        return (5, self.nz + 2 * g) + plane if self.nz > 1 else (5,) + plane
+
     def fixed_full(self):
-        """The ghost-inclusive state [5][NY+2g][NX+2g] the setup enrolls as fixed-face data, or None."""
+        """The ghost-inclusive state (ghost_shape()) the setup enrolls as fixed-face data, or None."""
         if len(self.fixed) == 0:
             return None
-        shape = (5, self.ny + 2 * self.ng, self.nx + 2 * self.ng)
+        shape = self.ghost_shape()
         f = np.asarray(self.fixed, np.float64)
         if f.ndim == 1:
-            return np.broadcast_to(f[:, None, None], shape).copy()
+            return np.broadcast_to(f.reshape((5,) + (1,) * (len(shape) - 1)), shape).copy()
         if f.shape != shape:
             raise ValueError(f"fixed-face data has shape {f.shape}, expected {shape}")
         return f.copy()
@@ -97,7 +104,7 @@ class Problem:
         return tuple(bc)
 
     def enrolled_state(self):
-        """The ghost-inclusive state [5][NY+2g][NX+2g] the setup holds when it enrolls its fixed data
+        """The ghost-inclusive state (ghost_shape()) the setup holds when it enrolls its fixed data
         (active cells = cons; ghosts of type-4 faces from `fixed`; ghosts of self-defined faces hold
         their state), or None when nothing is enrolled."""
         if not (len(self.fixed) or self.self_faces or self.fix_inner):
@@ -105,7 +112,12 @@ class Problem:
         g = self.ng
         full = self.fixed_full()
         if full is None:
-            full = np.zeros((5, self.ny + 2 * g, self.nx + 2 * g))
+            full = np.zeros(self.ghost_shape())
+        if self.nz > 1:
+            if self.self_faces:
+                raise ValueError("self-defined faces are given for 2D problems only")
+            full[:, g:g + self.nz, g:g + self.ny, g:g + self.nx] = self.cons
+            return full
         for f, state in self.self_faces:
             st = np.asarray(state, np.float64)[:, None, None]
             if f == 0:
@@ -137,7 +149,9 @@ class Problem:
                 for q, v in enumerate(self.fixed):
                     keys[f"fix_{q}"] = v
             else:
-                keys["fixedbd"] = 2   # the array comes from the file named by $GK_FIXED_FILE
+                # the array comes from the file named by $GK_FIXED_FILE: one plane (2D) or the
+                # whole ghost-inclusive volume (3D)
+                keys["fixedbd"] = 3 if np.ndim(self.fixed) == 4 else 2
         if not self.point:
             return keys
         keys["npoint"] = len(self.point)

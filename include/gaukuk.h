@@ -47,7 +47,7 @@ typedef struct {
 
 /* Physics and integrator: gamma (src/eos/adiabatic.cpp:12-16), CFL, integrator 1/2/3 and
  * RcOrder 1/2 (src/sim.cpp:47,56-66), stepmax, boundary codes xleft..zright: 0 copy, 1 periodic,
- * 2 reflective, 3 outflow, 4 fixed (2D x and y faces; its data comes from gk_set_fixed_boundary)
+ * 2 reflective, 3 outflow, 4 fixed (its data comes from gk_set_fixed_boundary)
  * (src/boundary/boundary.cpp:13-125), constant gravity (src/source_term/source.cpp:23-31),
  * precision 64 or 32 (the reference's Real, src/gaukuk.hpp:5-9). */
 typedef struct {
@@ -69,10 +69,11 @@ gk_status gk_set_point_gravity(gk_ctx* ctx, double gm, double x, double y, doubl
                                double rs);
 
 /* Boundary::EnrollFixedBoundaryData(cons, grid, fixInner) (src/boundary/fixed.cpp:242-269; faces
- * FixedBDXL..FixedBDYR, fixed.cpp:16-123): `cons` is a state in the reference layout and the
+ * FixedBDXL..FixedBDZR, fixed.cpp:16-179): `cons` is a state in the reference layout and the
  * context's precision; its ghost layers become the values that type-4 faces restore before every
- * stage.  With fix_inner != 0 its active cells inside the inner region (the central half-box, cell
- * centres within 0.5 of the origin: Boundary::FixedBDInner, fixed.cpp:181-207) are restored
+ * stage (x and y faces on every active plane, z faces over the active rows).  With fix_inner != 0
+ * its active cells inside the inner region (the central half-box in x and y on every active plane,
+ * cell centres within 0.5 of the origin: Boundary::FixedBDInner, fixed.cpp:181-207) are restored
  * there before every stage as well.  Advancing a context with a type-4 face fails with GK_ESTATE
  * until this is called, as the reference throws. */
 gk_status gk_set_fixed_boundary(gk_ctx* ctx, const void* cons, int fix_inner);

@@ -246,36 +246,50 @@ struct Oracle {
         fixedSnap = cons;
         if (fixInner) fixedInner = cons;
     }
+    // every active plane k (fixed.cpp: k in [kb, ke))
     void FillFixedX(std::vector<Real>& U, bool left) {
         if (fixedSnap.empty())
             throw std::invalid_argument("fixed boundary is not initialized");
         const int gl = left ? 0 : ie, gr = left ? ib : lenx;
         for (int n = 0; n < NV; n++)
-            for (int j = jb; j < je; j++)
-                for (int i = gl; i < gr; i++) U[at(n, j, i)] = fixedSnap[at(n, j, i)];
+            for (int k = kb; k < ke; k++)
+                for (int j = jb; j < je; j++)
+                    for (int i = gl; i < gr; i++) U[at(n, j, i, k)] = fixedSnap[at(n, j, i, k)];
     }
     void FillFixedY(std::vector<Real>& U, bool left) {
         if (fixedSnap.empty())
             throw std::invalid_argument("fixed boundary is not initialized");
         const int gl = left ? 0 : je, gr = left ? jb : leny;
         for (int n = 0; n < NV; n++)
-            for (int j = gl; j < gr; j++)
-                for (int i = ib; i < ie; i++) U[at(n, j, i)] = fixedSnap[at(n, j, i)];
+            for (int k = kb; k < ke; k++)
+                for (int j = gl; j < gr; j++)
+                    for (int i = ib; i < ie; i++) U[at(n, j, i, k)] = fixedSnap[at(n, j, i, k)];
     }
-    // Inner fixed region: src/boundary/fixed.cpp:181-207.  Cells of the central half-box whose
-    // centre lies within 0.5 of the origin take the saved state.
+    // z faces (3D): the saved ghost planes over the active (j, i) range (fixed.cpp:126-179)
+    void FillFixedZ(std::vector<Real>& U, bool left) {
+        if (fixedSnap.empty())
+            throw std::invalid_argument("fixed boundary is not initialized");
+        const int gl = left ? 0 : ke, gr = left ? kb : lenz;
+        for (int n = 0; n < NV; n++)
+            for (int k = gl; k < gr; k++)
+                for (int j = jb; j < je; j++)
+                    for (int i = ib; i < ie; i++) U[at(n, j, i, k)] = fixedSnap[at(n, j, i, k)];
+    }
+    // Inner fixed region: src/boundary/fixed.cpp:181-207.  Cells of the central half-box in x and
+    // y, on every active plane, whose centre lies within 0.5 of the origin take the saved state.
     void FixedBDInner(std::vector<Real>& U) {
         const int il = static_cast<int>(ib + 0.25 * (ie - ib));
         const int ir = static_cast<int>(ib + 0.75 * (ie - ib));
         const int jl = static_cast<int>(jb + 0.25 * (je - jb));
         const int jr = static_cast<int>(jb + 0.75 * (je - jb));
         for (int n = 0; n < NV; n++)
-            for (int j = jl; j < jr; j++)
-                for (int i = il; i < ir; i++) {
-                    const Real xnow = xc[i], ynow = yc[j], znow = zc0;
-                    const Real r2 = xnow * xnow + ynow * ynow + znow * znow;
-                    if (r2 < 0.5 * 0.5) U[at(n, j, i)] = fixedInner[at(n, j, i)];
-                }
+            for (int k = kb; k < ke; k++)
+                for (int j = jl; j < jr; j++)
+                    for (int i = il; i < ir; i++) {
+                        const Real xnow = xc[i], ynow = yc[j], znow = nz > 1 ? zc[k] : zc0;
+                        const Real r2 = xnow * xnow + ynow * ynow + znow * znow;
+                        if (r2 < 0.5 * 0.5) U[at(n, j, i, k)] = fixedInner[at(n, j, i, k)];
+                    }
     }
 
     // ---- Immersed reflective shell: src/boundary/immersed.cpp -----------------------------
@@ -423,16 +437,14 @@ struct Oracle {
     // Boundary::UpdateBD: src/boundary/boundary.cpp:127-142 -- faces, then the immersed shell,
     // then the inner fixed region
     void UpdateBD(std::vector<Real>& U) {
-        if (nz > 1 && (P.bc[0] == 4 || P.bc[1] == 4 || P.bc[2] == 4 || P.bc[3] == 4 || P.bc[4] == 4 ||
-                       P.bc[5] == 4 || immersed || !fixedInner.empty()))
-            throw std::invalid_argument("3D runs take face types 0-3 only here");
+        if (nz > 1 && immersed) throw std::invalid_argument("3D runs take no immersed shell here");
         if (P.bc[0] == 4) FillFixedX(U, true); else FillX(U, true, P.bc[0]);
         if (P.bc[1] == 4) FillFixedX(U, false); else FillX(U, false, P.bc[1]);
         if (P.bc[2] == 4) FillFixedY(U, true); else FillY(U, true, P.bc[2]);
         if (P.bc[3] == 4) FillFixedY(U, false); else FillY(U, false, P.bc[3]);
         if (nz > 1) {
-            FillZ(U, true, P.bc[4]);
-            FillZ(U, false, P.bc[5]);
+            if (P.bc[4] == 4) FillFixedZ(U, true); else FillZ(U, true, P.bc[4]);
+            if (P.bc[5] == 4) FillFixedZ(U, false); else FillZ(U, false, P.bc[5]);
         }
         if (immersed) ImmersedReflective(U);
         if (!fixedInner.empty()) FixedBDInner(U);

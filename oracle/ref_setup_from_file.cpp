@@ -11,7 +11,8 @@
 // pg<q>_gm, _x, _y, _z, _vx, _vy, _vz, _rs enroll point masses in order
 // (SourceTerm::EnrollPointGravity, src/source_term/source.cpp:33-67; two masses = binary gravity);
 // fixedbd = 1 fills every ghost cell with the conserved state fix_0..fix_4, fixedbd = 2 with the
-// plane read from $GK_FIXED_FILE, and enrolls the fixed boundary data
+// plane read from $GK_FIXED_FILE, fixedbd = 3 with the whole ghost-inclusive 3D array read from
+// $GK_FIXED_FILE, and enrolls the fixed boundary data
 // (Boundary::EnrollFixedBoundaryData, src/boundary/fixed.cpp:242-269); fixinner = 1 enrolls it with
 // the inner region (fixed.cpp:181-207); selfbd_<f> = 1 enrolls a self-defined face f (0 xleft,
 // 1 xright, 2 yleft, 3 yright) whose ghost cells take the conserved state sd<f>_0..sd<f>_4 (the
@@ -53,6 +54,31 @@ static void ReadFixedPlane(TArray<Real>& cons, const Grid& grid) {
                     cons(v, k, j, i) = (Real)buf[((size_t)v * grid.leny + j) * grid.lenx + i];
 }
 
+// fixedbd = 3: the ghost-inclusive array [5][NZ+2g][NY+2g][NX+2g] from $GK_FIXED_FILE
+// (int32 nvar, lenx, leny, lenz; float64 data)
+static void ReadFixedVolume(TArray<Real>& cons, const Grid& grid) {
+    const char* path = std::getenv("GK_FIXED_FILE");
+    if (!path) throw std::runtime_error("fixedbd = 3 needs GK_FIXED_FILE");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    int hdr[4] = {0, 0, 0, 0};
+    if (std::fread(hdr, sizeof(int), 4, f) != 4 || hdr[0] != NVar || hdr[1] != grid.lenx || hdr[2] != grid.leny ||
+        hdr[3] != grid.lenz) {
+        std::fclose(f);
+        throw std::runtime_error("fixed-data file header does not match the ghost-inclusive grid");
+    }
+    const size_t n = (size_t)NVar * grid.lenx * grid.leny * grid.lenz;
+    std::vector<double> buf(n);
+    const size_t got = std::fread(buf.data(), sizeof(double), n, f);
+    std::fclose(f);
+    if (got != n) throw std::runtime_error("short fixed data");
+    for (int v = 0; v < NVar; v++)
+        for (int k = 0; k < grid.lenz; k++)
+            for (int j = 0; j < grid.leny; j++)
+                for (int i = 0; i < grid.lenx; i++)
+                    cons(v, k, j, i) = (Real)buf[(((size_t)v * grid.lenz + k) * grid.leny + j) * grid.lenx + i];
+}
+
 // a self-defined face writing one conserved state into its ghost cells (active tangential range)
 static void ConstantFace(TArray<Real>& cons, const Grid& grid, int f) {
     Real v[NVar];
@@ -88,6 +114,7 @@ void Sim::Setup() {
     std::fclose(f);
     const int fixedbd = (int)Config::getInstance().get("fixedbd", 0.0);
     if (fixedbd == 2) ReadFixedPlane(cons, grid);
+    if (fixedbd == 3) ReadFixedVolume(cons, grid);
     if (fixedbd == 1) {
         for (int v = 0; v < NVar; v++) {
             const Real fv = (Real)Config::getInstance().get("fix_" + std::to_string(v), 0.0);
@@ -108,7 +135,7 @@ void Sim::Setup() {
     srcTerm.EnrollConstGravityVector(gx, gy, gz);
     Config& c = Config::getInstance();
     const bool fixinner = (int)c.get("fixinner", 0.0) == 1;
-    if (fixedbd == 1 || fixedbd == 2 || fixinner) boundary.EnrollFixedBoundaryData(cons, grid, fixinner);
+    if ((fixedbd >= 1 && fixedbd <= 3) || fixinner) boundary.EnrollFixedBoundaryData(cons, grid, fixinner);
     if ((int)c.get("selfbd_0", 0.0) == 1) boundary.EnrollSelfDefineBDXL();
     if ((int)c.get("selfbd_1", 0.0) == 1) boundary.EnrollSelfDefineBDXR();
     if ((int)c.get("selfbd_2", 0.0) == 1) boundary.EnrollSelfDefineBDYL();

@@ -34,6 +34,11 @@ def run_from_file(workdir, prob, extra_args=(), in_values=None, timeout=600):
     if np.ndim(prob.fixed) == 3:
         env["GK_FIXED_FILE"] = os.path.join(workdir, "fixed.bin")
         write_ic(env["GK_FIXED_FILE"], prob.fixed)
+    elif np.ndim(prob.fixed) == 4:
+        env["GK_FIXED_FILE"] = os.path.join(workdir, "fixed.bin")
+        with open(env["GK_FIXED_FILE"], "wb") as f:
+            np.array([5, prob.fixed.shape[3], prob.fixed.shape[2], prob.fixed.shape[1]], np.int32).tofile(f)
+            np.ascontiguousarray(prob.fixed, np.float64).tofile(f)
     r = subprocess.run([driver("from_file"), "-i", inp, *extra_args], cwd=workdir, env=env,
                        capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:

@@ -39,7 +39,7 @@ def _fixed(z, m):
     kind = int(m.get("fixedbd", 0))
     if kind == 1:
         return tuple(m[f"fix_{q}"] for q in range(5))
-    if kind == 2:
+    if kind in (2, 3):
         return z["fixed_full"].copy()
     return ()
 

@@ -105,6 +105,25 @@ def cases():
     out["blast3d_z0_rk2"] = P.blast3d(16, 20, 14, bc=(1, 1, 3, 3), bcz=(0, 0), stepmax=15, center=(0.4, 0.5, 0.3))
     out["blast3d_fe_rc1"] = P.blast3d(20, 16, 12, bc=(1, 3, 2, 1), bcz=(1, 1), stepmax=10).with_(
         integrator=1, rc_order=1)
+    # 3D fixed faces (src/boundary/fixed.cpp:126-179, 242-269): type 4 on one face of every axis,
+    # every ghost cell saved with its own value, constant gravity on two axes (RK3) -- pins which
+    # saved layer each x, y and z face restores and over which range; then all six faces type 4
+    # with one inflow state (RK2)
+    f3 = P.blast3d(20, 18, 16, bc=(4, 3, 2, 4), bcz=(4, 3), stepmax=15, center=(0.45, 0.5, 0.4))
+    rng = np.random.default_rng(11)
+    shp = f3.ghost_shape()[1:]
+    f3snap = P.prim_to_cons(rng.uniform(0.8, 1.2, shp), rng.uniform(-0.3, 0.3, shp), rng.uniform(-0.3, 0.3, shp),
+                            rng.uniform(-0.3, 0.3, shp), rng.uniform(0.05, 0.15, shp), f3.gamma)
+    out["fixed3d_mixed_rk3_g"] = f3.with_(integrator=3, g=(0.0, -0.1, 0.05), fixed=f3snap)
+    f6 = P.blast3d(16, 18, 14, bc=(4, 4, 4, 4), bcz=(4, 4), stepmax=15, center=(0.5, 0.45, 0.4))
+    d0, vx0, vy0, vz0, p0 = 1.1, 0.3, -0.1, 0.2, 0.12
+    out["fixed3d_all4_rk2"] = f6.with_(fixed=(d0, d0 * vx0, d0 * vy0, d0 * vz0,
+                                              p0 / (f6.gamma - 1.0) + 0.5 * d0 * (vx0 ** 2 + vy0 ** 2 + vz0 ** 2)))
+    # 3D inner fixed region (src/boundary/fixed.cpp:181-207): the sphere r < 0.5 around the origin
+    # on every active plane, with a blast straddling it, periodic x, reflective y, outflow z, gravity
+    i3 = P.blast3d(24, 24, 12, bc=(1, 1, 2, 2), bcz=(3, 3), stepmax=15, radius=0.2, center=(0.6, 0.55, 0.3))
+    out["inner3d_rk3_g"] = i3.with_(extent=(-1.0, 1.0, -1.0, 1.0, -0.5, 0.5), integrator=3, g=(0.0, -0.1, 0.05),
+                                    fix_inner=True)
     return out
 
 
@@ -127,10 +146,12 @@ def run_reference(prob, workdir):
     inp = os.path.join(workdir, "case.in")
     write_in(inp, prob.in_values())
     env = dict(os.environ, GK_IC_FILE=ic, OMP_NUM_THREADS="4")
-    if np.ndim(prob.fixed) == 3:
+    if np.ndim(prob.fixed) in (3, 4):
         env["GK_FIXED_FILE"] = os.path.join(workdir, "fixed.bin")
+        g = prob.ng
+        hdr = [5, prob.nx + 2 * g, prob.ny + 2 * g] + ([prob.nz + 2 * g] if np.ndim(prob.fixed) == 4 else [])
         with open(env["GK_FIXED_FILE"], "wb") as f:
-            np.array([5, prob.nx + 2 * prob.ng, prob.ny + 2 * prob.ng], np.int32).tofile(f)
+            np.array(hdr, np.int32).tofile(f)
             np.ascontiguousarray(prob.fixed, np.float64).tofile(f)
     r = subprocess.run([os.path.join(REF, "gaukuk_ref"), "-i", inp], cwd=workdir, env=env,
                        capture_output=True, text=True, check=True)
@@ -335,7 +356,7 @@ def main():
         with tempfile.TemporaryDirectory() as d:
             final, t, steps, nout = run_reference(prob, d)
         meta = {k: v for k, v in prob.in_values().items()}
-        extra = {"fixed_full": prob.fixed} if np.ndim(prob.fixed) == 3 else {}
+        extra = {"fixed_full": prob.fixed} if np.ndim(prob.fixed) in (3, 4) else {}
         np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), ic=prob.cons, final=final,
                             t=np.array(t), steps=np.array(steps), meta=json.dumps(meta), **extra)
         print(f"{name:28s} {prob.nx}x{prob.ny} steps={steps} t={t!r}")

@@ -32,12 +32,10 @@ def test_create_rejects_bad_grid_without_gpu():
     from gaukuk_amd import problems as P
     prob = P.kelvin_helmholtz(8, seed=0)
     g = hip.make_grid(prob)
-    g.nz = 2
-    phys = hip.make_phys(prob)
-    phys.bc[4] = 4   # fixed faces are 2D only: rejected before any device work
+    g.nz, g.ny = 2, 1   # a 3D grid of one row: rejected before any device work
     h = ctypes.c_void_p()
-    st = hip.lib().gk_create(ctypes.byref(g), ctypes.byref(phys), ctypes.byref(h))
-    assert st == 1 and b"3D grids take face types 0-3" in hip.lib().gk_last_error(None)
-    g.nz, g.nghost = 1, 1
+    st = hip.lib().gk_create(ctypes.byref(g), ctypes.byref(hip.make_phys(prob)), ctypes.byref(h))
+    assert st == 1 and b"3D grids need NY >= 2" in hip.lib().gk_last_error(None)
+    g.nz, g.ny, g.nghost = 1, 8, 1
     st = hip.lib().gk_create(ctypes.byref(g), ctypes.byref(hip.make_phys(prob)), ctypes.byref(h))
     assert st == 1 and b"NGhost" in hip.lib().gk_last_error(None)
