@@ -737,7 +737,7 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
         g_create_error = "Invalid grid parameters from config"; return GK_EINVAL;
     }
     if (grid->nz > 1) {
-        // 3D: immersed shells, boundary hooks, ensembles and the learned correction are 2D only
+        // 3D: boundary hooks, ensembles and the learned correction are 2D only
         if (grid->ny < 2) { g_create_error = "3D grids need NY >= 2"; return GK_EINVAL; }
     }
     if (grid->nghost < 2) { g_create_error = "NGhost must be >= 2 for the PLM stencil"; return GK_EINVAL; }
@@ -1085,16 +1085,19 @@ gk_status gk_set_fixed_boundary(gk_ctx* c, const void* host, int fix_inner) {
 gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, double rs) {
     if (!c) return GK_EINVAL;
     if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
-    // Boundary::SetupImmersedShell (src/boundary/immersed.cpp:89-235), nz == 1
+    // Boundary::SetupImmersedShell (src/boundary/immersed.cpp:89-235); in 2D the single plane
     if (c->nx <= 1 || c->ny <= 1) return fail(c, GK_EINVAL, "immersed boundary has to be at least 2D. ");
-    if (c->nz > 1) return fail(c, GK_EINVAL, "immersed shells are 2D only");
-    const int g = c->grid.nghost, ib = g, ie = g + c->nx, jb = g, je = g + c->ny;
+    const bool three = c->nz > 1;
+    const int g = c->grid.nghost, ib = g, ie = g + c->nx, jb = g, je = g + c->ny, kb = g, ke = g + c->nz;
     std::vector<double> xc, yc;
-    double zc;
-    domain_centres(c, xc, yc, zc);
+    double zc0;
+    domain_centres(c, xc, yc, zc0);
     const double dx = (c->grid.xmax - c->grid.xmin) / c->grid.nx;
     const double dy = (c->grid.ymax - c->grid.ymin) / c->grid.ny;
     const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
+    // z centres, ghost-inclusive index (Domain, src/sim.cpp:39); 2D uses the active plane's
+    std::vector<double> zc((size_t)c->nz + 2 * g);
+    for (int k = 0; k < (int)zc.size(); k++) zc[k] = three ? c->grid.zmin - g * dz + 0.5 * dz + k * dz : zc0;
     // first cell at or past the body's extent from the low end / last at or before it from the
     // high end (FindLeftId, FindRightId: immersed.cpp:19-47)
     bool outside = false;
@@ -1114,30 +1117,40 @@ gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, doub
     };
     const int il = first_from(xc, ib, ie, xs, c->grid.xmin), ir = last_to(xc, ib, ie, xs, c->grid.xmax);
     const int jl = first_from(yc, jb, je, ys, c->grid.ymin), jr = last_to(yc, jb, je, ys, c->grid.ymax);
-    if (outside || il > ir || jl > jr) return fail(c, GK_EINVAL, "immersed boundary out of domain");
+    int kl = kb, kr = ke;
+    if (three) {
+        kl = first_from(zc, kb, ke, zs, c->grid.zmin);
+        kr = last_to(zc, kb, ke, zs, c->grid.zmax);
+    }
+    if (outside || il > ir || jl > jr || kl > kr) return fail(c, GK_EINVAL, "immersed boundary out of domain");
+    // state offset of ghost-inclusive cell (i, j, k): planes of ny rows follow each other
+    auto offset = [&](int i, int j, int k) {
+        return (long long)(k - g) * c->plane2 + (long long)(j - g) * c->pitch + (i - g);
+    };
     // shell cells: inside the body with a neighbour nGhost cells away (in one axis) outside it
     // (IsShellCell: immersed.cpp:49-69)
     const double sx = g * dx, sy = g * dy, sz = g * dz;
     auto outside_at = [&](double ax, double ay, double az) { return std::sqrt(ax * ax + ay * ay + az * az) > rs; };
-    std::vector<int> si, sj;
-    for (int j = jl; j < jr; j++)
-        for (int i = il; i < ir; i++) {
-            const double ax = xc[i] - xs, ay = yc[j] - ys, az = zc - zs;
-            if (!(std::sqrt(ax * ax + ay * ay + az * az) < rs)) continue;
-            if (outside_at(ax + sx, ay, az) || outside_at(ax - sx, ay, az) || outside_at(ax, ay + sy, az) ||
-                outside_at(ax, ay - sy, az) || outside_at(ax, ay, az + sz) || outside_at(ax, ay, az - sz)) {
-                si.push_back(i); sj.push_back(j);
+    std::vector<int> si, sj, sk;
+    for (int k = kl; k < kr; k++)
+        for (int j = jl; j < jr; j++)
+            for (int i = il; i < ir; i++) {
+                const double ax = xc[i] - xs, ay = yc[j] - ys, az = zc[k] - zs;
+                if (!(std::sqrt(ax * ax + ay * ay + az * az) < rs)) continue;
+                if (outside_at(ax + sx, ay, az) || outside_at(ax - sx, ay, az) || outside_at(ax, ay + sy, az) ||
+                    outside_at(ax, ay - sy, az) || outside_at(ax, ay, az + sz) || outside_at(ax, ay, az - sz)) {
+                    si.push_back(i); sj.push_back(j); sk.push_back(k);
+                }
             }
-        }
     const int ns = (int)si.size();
     std::vector<long long> cell(ns), tar((size_t)kShellTargets * ns, 0);
     std::vector<int> ntar(ns);
     std::vector<double> dir((size_t)4 * ns), gam((size_t)(kShellTargets + 1) * ns, 0.0);
-    const int reach = 2 * g;
+    const int reach = 2 * g, zreach = three ? 2 * g : 0;
     for (int s = 0; s < ns; s++) {
-        const int i = si[s], j = sj[s];
-        cell[s] = (long long)(j - g) * c->pitch + (i - g);
-        const double ax = xc[i] - xs, ay = yc[j] - ys, az = zc - zs;
+        const int i = si[s], j = sj[s], k = sk[s];
+        cell[s] = offset(i, j, k);
+        const double ax = xc[i] - xs, ay = yc[j] - ys, az = zc[k] - zs;
         const double rnow = std::sqrt(ax * ax + ay * ay + az * az);
         const double rxy = std::sqrt(ax * ax + ay * ay);
         double cosp = 1.0, sinp = 0.0, cost = 1.0, sint = 0.0;
@@ -1152,19 +1165,21 @@ gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, doub
         dist[0] = 2 * (rs - rnow);
         int nt = 0;
         for (int di = -reach; di <= reach; di++)
-            for (int dj = -reach; dj <= reach; dj++) {
-                const int ti = i + di, tj = j + dj;
-                if (ti < ib || ti >= ie || tj < jb || tj >= je)
-                    return fail(c, GK_EINVAL, "immersed body is too close to the domain boundary, "
-                                              "neighboor search out of range.");
-                const double tx = xc[ti], ty = yc[tj], tz = zc;
-                if (!(std::abs(tx - xi) <= dx && std::abs(ty - yi) <= dy && std::abs(tz - zi) <= dz)) continue;
-                const double rt = std::sqrt((tx - xs) * (tx - xs) + (ty - ys) * (ty - ys) + (tz - zs) * (tz - zs));
-                if (!(rt > rs)) continue;
-                if (++nt > kShellTargets) return fail(c, GK_EINVAL, "n of imagine points is more than nTarMax");
-                tar[(size_t)(nt - 1) * ns + s] = (long long)(tj - g) * c->pitch + (ti - g);
-                dist[nt] = std::sqrt((tx - xi) * (tx - xi) + (ty - yi) * (ty - yi) + (tz - zi) * (tz - zi));
-            }
+            for (int dj = -reach; dj <= reach; dj++)
+                for (int dk = -zreach; dk <= zreach; dk++) {
+                    const int ti = i + di, tj = j + dj, tk = k + dk;
+                    if (ti < ib || ti >= ie || tj < jb || tj >= je || tk < kb || tk >= ke)
+                        return fail(c, GK_EINVAL, "immersed body is too close to the domain boundary, "
+                                                  "neighboor search out of range.");
+                    const double tx = xc[ti], ty = yc[tj], tz = zc[tk];
+                    if (!(std::abs(tx - xi) <= dx && std::abs(ty - yi) <= dy && std::abs(tz - zi) <= dz)) continue;
+                    const double rt =
+                        std::sqrt((tx - xs) * (tx - xs) + (ty - ys) * (ty - ys) + (tz - zs) * (tz - zs));
+                    if (!(rt > rs)) continue;
+                    if (++nt > kShellTargets) return fail(c, GK_EINVAL, "n of imagine points is more than nTarMax");
+                    tar[(size_t)(nt - 1) * ns + s] = offset(ti, tj, tk);
+                    dist[nt] = std::sqrt((tx - xi) * (tx - xi) + (ty - yi) * (ty - yi) + (tz - zi) * (tz - zi));
+                }
         ntar[s] = nt;
         // inverse-square distance weights normalised over the cell and its targets
         double sum = 0.;

@@ -69,7 +69,7 @@ public:
 
     // Fixed faces (src/boundary/fixed.cpp:242-269): keeps a copy of `cons`, whose ghost layers
     // type-4 faces restore; with fixInner the inner region's active cells are restored before every
-    // stage as well (src/boundary/boundary.cpp:139-141, fixed.cpp:181-207; 2D on the device).
+    // stage as well (src/boundary/boundary.cpp:139-141, fixed.cpp:181-207).
     void EnrollFixedBoundaryData(const TArray<Real>& cons, const Grid& grid, bool fixInner = false);
     bool AnyFixed() const;
     bool FixedEnrolled() const { return fixed_->enrolled; }

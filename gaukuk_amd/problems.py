@@ -309,6 +309,22 @@ def blast3d(nx: int, ny: int, nz: int, p_ratio: float = 30.0, radius: float = 0.
     return Problem(cons=prim_to_cons(rho, zero, zero, zero, p, base.get("gamma", 1.4)), **base)
 
 
+def sphere_flow3d(nx: int, ny: int, nz: int, **kw) -> Problem:
+    """An immersed sphere of radius 0.2 at (0.55, 0.45, 0.4) on [0, nx/L] x [0, ny/L] x [0, nz/L],
+    L = max(nx, ny, nz), in the oblique flow (0.5, 0.1, -0.05) with an overpressure blob upstream;
+    periodic x and z, reflective y left, outflow y right, RK3."""
+    prob = blast3d(nx, ny, nz, bc=(1, 1, 2, 3), bcz=(1, 1))
+    L = float(max(nx, ny, nz))
+    Z, Y, X = np.meshgrid((np.arange(nz) + 0.5) / L, (np.arange(ny) + 0.5) / L, (np.arange(nx) + 0.5) / L,
+                          indexing="ij")
+    one = np.ones_like(X)
+    p = np.where((X - 0.2) ** 2 + (Y - 0.45) ** 2 + (Z - 0.4) ** 2 < 0.01, 1.5, 0.6)
+    cons = prim_to_cons(1.0 + 0.1 * X, 0.5 * one, 0.1 * one, -0.05 * one, p, prob.gamma)
+    base = dict(name="sphere3d", integrator=3, cons=cons, shell=(0.55, 0.45, 0.4, 0.2))
+    base.update(kw)
+    return prob.with_(**base)
+
+
 def shear3d(n: int, nz: int, seed: int = 0, amp: float = 0.02, **kw) -> Problem:
     """3D Kelvin-Helmholtz: the configuration's shear layer (rho 2 / 1, vx -+0.5, p 2.5) on
     [0,1]^2 x [0, nz/n], with a seeded perturbation in vy and vz."""

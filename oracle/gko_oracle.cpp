@@ -295,8 +295,8 @@ struct Oracle {
     // ---- Immersed reflective shell: src/boundary/immersed.cpp -----------------------------
     static constexpr int nTarMax = 8;
     struct Shell {
-        int i, j, nTar;
-        int ti[nTarMax + 1], tj[nTarMax + 1];
+        int i, j, k, nTar;
+        int ti[nTarMax + 1], tj[nTarMax + 1], tk[nTarMax + 1];
         Real cosp, sinp, cost, sint;
         Real gam[nTarMax + 1];
     };
@@ -330,9 +330,10 @@ struct Oracle {
         }
         return false;
     }
-    // Boundary::SetupImmersedShell: immersed.cpp:89-235 (nz == 1: the active z plane only).  The
-    // shell cells are listed in (j, i) order; the reference's order depends on its threads, and the
+    // Boundary::SetupImmersedShell: immersed.cpp:89-235 (2D: the active z plane only).  The shell
+    // cells are listed in (k, j, i) order; the reference's order depends on its threads, and the
     // result does not (shell cells and their targets are disjoint).
+    Real zcAt(int k) const { return nz > 1 ? zc[k] : zc0; }
     void SetupImmersedShell(Real xs, Real ys, Real zs, Real rs) {
         if (nx <= 1 || ny <= 1) throw std::invalid_argument("immersed boundary has to be at least 2D. ");
         immersed = true;
@@ -340,20 +341,26 @@ struct Oracle {
         const int ir = FindRightId(xc, ib, ie, xs, rs, (Real)P.xmax);
         const int jl = FindLeftId(yc, jb, je, ys, rs, (Real)P.ymin);
         const int jr = FindRightId(yc, jb, je, ys, rs, (Real)P.ymax);
-        if (il > ir || jl > jr) throw std::invalid_argument("immersed boundary out of domain");
+        int kl = kb, kr = ke;
+        if (nz > 1) {
+            kl = FindLeftId(zc, kb, ke, zs, rs, (Real)P.zmin);
+            kr = FindRightId(zc, kb, ke, zs, rs, (Real)P.zmax);
+        }
+        if (il > ir || jl > jr || kl > kr) throw std::invalid_argument("immersed boundary out of domain");
         shells.clear();
         const Real sx = ng * dx, sy = ng * dy, sz = ng * dz;
-        for (int j = jl; j < jr; j++)
-            for (int i = il; i < ir; i++)
-                if (IsShellCell(xc[i] - xs, yc[j] - ys, zc0 - zs, rs, sx, sy, sz)) {
-                    Shell s{};
-                    s.i = i; s.j = j;
-                    shells.push_back(s);
-                }
-        const int nxs = 2 * ng, nys = 2 * ng;
+        for (int k = kl; k < kr; k++)
+            for (int j = jl; j < jr; j++)
+                for (int i = il; i < ir; i++)
+                    if (IsShellCell(xc[i] - xs, yc[j] - ys, zcAt(k) - zs, rs, sx, sy, sz)) {
+                        Shell s{};
+                        s.i = i; s.j = j; s.k = k;
+                        shells.push_back(s);
+                    }
+        const int nxs = 2 * ng, nys = 2 * ng, nzs = nz > 1 ? 2 * ng : 0;
         for (Shell& s : shells) {
-            s.ti[0] = s.i; s.tj[0] = s.j;
-            const Real delx = xc[s.i] - xs, dely = yc[s.j] - ys, delz = zc0 - zs;
+            s.ti[0] = s.i; s.tj[0] = s.j; s.tk[0] = s.k;
+            const Real delx = xc[s.i] - xs, dely = yc[s.j] - ys, delz = zcAt(s.k) - zs;
             const Real rnow = std::sqrt(delx * delx + dely * dely + delz * delz);
             Real cosp, sinp, cost, sint;
             const Real rxy = std::sqrt(delx * delx + dely * dely);
@@ -367,23 +374,25 @@ struct Oracle {
             Real disDel[nTarMax + 1];
             disDel[0] = 2 * (rs - rnow);
             for (int ii = -nxs; ii < nxs + 1; ++ii)
-                for (int jj = -nys; jj < nys + 1; ++jj) {
-                    const int iT = s.i + ii, jT = s.j + jj;
-                    if (iT < ib || iT >= ie || jT < jb || jT >= je)
-                        throw std::invalid_argument("immersed body is too close to the domain boundary, "
-                                                    "neighboor search out of range.");
-                    const Real xT = xc[iT], yT = yc[jT], zT = zc0;
-                    if (std::abs(xT - xImg) <= dx && std::abs(yT - yImg) <= dy && std::abs(zT - zImg) <= dz) {
-                        const Real rT = std::sqrt((xT - xs) * (xT - xs) + (yT - ys) * (yT - ys) + (zT - zs) * (zT - zs));
-                        if (rT > rs) {
-                            nt++;
-                            if (nt > nTarMax) throw std::logic_error("n of imagine points is more than nTarMax");
-                            s.ti[nt] = iT; s.tj[nt] = jT;
-                            disDel[nt] = std::sqrt((xT - xImg) * (xT - xImg) + (yT - yImg) * (yT - yImg) +
-                                                   (zT - zImg) * (zT - zImg));
+                for (int jj = -nys; jj < nys + 1; ++jj)
+                    for (int kk = -nzs; kk < nzs + 1; ++kk) {
+                        const int iT = s.i + ii, jT = s.j + jj, kT = s.k + kk;
+                        if (iT < ib || iT >= ie || jT < jb || jT >= je || kT < kb || kT >= ke)
+                            throw std::invalid_argument("immersed body is too close to the domain boundary, "
+                                                        "neighboor search out of range.");
+                        const Real xT = xc[iT], yT = yc[jT], zT = zcAt(kT);
+                        if (std::abs(xT - xImg) <= dx && std::abs(yT - yImg) <= dy && std::abs(zT - zImg) <= dz) {
+                            const Real rT =
+                                std::sqrt((xT - xs) * (xT - xs) + (yT - ys) * (yT - ys) + (zT - zs) * (zT - zs));
+                            if (rT > rs) {
+                                nt++;
+                                if (nt > nTarMax) throw std::logic_error("n of imagine points is more than nTarMax");
+                                s.ti[nt] = iT; s.tj[nt] = jT; s.tk[nt] = kT;
+                                disDel[nt] = std::sqrt((xT - xImg) * (xT - xImg) + (yT - yImg) * (yT - yImg) +
+                                                       (zT - zImg) * (zT - zImg));
+                            }
                         }
                     }
-                }
             s.nTar = nt;
             Real sum = 0.;
             for (int n = 0; n < nt + 1; n++) sum = sum + 1. / (disDel[n] * disDel[n]);
@@ -400,14 +409,14 @@ struct Oracle {
             Real dens = 0, vr = 0, vp = 0, vt = 0, pres = 0;
             const Real g0 = s.gam[0];
             for (int n = 1; n <= s.nTar; n++) {
-                const int iT = s.ti[n], jT = s.tj[n];
-                Real denTar = U[at(DEN, jT, iT)];
+                const int iT = s.ti[n], jT = s.tj[n], kT = s.tk[n];
+                Real denTar = U[at(DEN, jT, iT, kT)];
                 denTar = std::max(denTar, dmin);
                 const Real denInv = 1.0 / denTar;
                 dens += 1. / (1 - g0) * s.gam[n] * denTar;
-                const Real vx = U[at(MX, jT, iT)] * denInv;
-                const Real vy = U[at(MY, jT, iT)] * denInv;
-                const Real vz = U[at(MZ, jT, iT)] * denInv;
+                const Real vx = U[at(MX, jT, iT, kT)] * denInv;
+                const Real vy = U[at(MY, jT, iT, kT)] * denInv;
+                const Real vz = U[at(MZ, jT, iT, kT)] * denInv;
                 // CarToLocal: immersed.cpp:73-79
                 const Real vrl = vx * cosp * sint + vy * sinp * sint + vz * cost;
                 const Real vpl = -vx * sinp + vy * cosp;
@@ -415,7 +424,7 @@ struct Oracle {
                 vr += 1. / (-1. - g0) * s.gam[n] * vrl;
                 vp += 1. / (1. - g0) * s.gam[n] * vpl;
                 vt += 1. / (1. - g0) * s.gam[n] * vtl;
-                const Real engTar = U[at(ENG, jT, iT)];
+                const Real engTar = U[at(ENG, jT, iT, kT)];
                 const Real engKin = 0.5 * denTar * (vx * vx + vy * vy + vz * vz);
                 Real preTar = gm1 * (engTar - engKin);
                 preTar = (preTar > pmin) ? preTar : pmin;
@@ -425,19 +434,18 @@ struct Oracle {
             const Real vx = vr * sint * cosp + vt * cost * cosp - vp * sinp;
             const Real vy = vr * sint * sinp + vt * cost * sinp + vp * cosp;
             const Real vz = vr * cost - vt * sint;
-            U[at(DEN, s.j, s.i)] = dens;
-            U[at(MX, s.j, s.i)] = dens * vx;
-            U[at(MY, s.j, s.i)] = dens * vy;
-            U[at(MZ, s.j, s.i)] = dens * vz;
+            U[at(DEN, s.j, s.i, s.k)] = dens;
+            U[at(MX, s.j, s.i, s.k)] = dens * vx;
+            U[at(MY, s.j, s.i, s.k)] = dens * vy;
+            U[at(MZ, s.j, s.i, s.k)] = dens * vz;
             const Real engKin = 0.5 * dens * (vx * vx + vy * vy + vz * vz);
-            U[at(ENG, s.j, s.i)] = engKin + pres * gm1Rec;   // EGas: eos.hpp:30-32
+            U[at(ENG, s.j, s.i, s.k)] = engKin + pres * gm1Rec;   // EGas: eos.hpp:30-32
         }
     }
 
     // Boundary::UpdateBD: src/boundary/boundary.cpp:127-142 -- faces, then the immersed shell,
     // then the inner fixed region
     void UpdateBD(std::vector<Real>& U) {
-        if (nz > 1 && immersed) throw std::invalid_argument("3D runs take no immersed shell here");
         if (P.bc[0] == 4) FillFixedX(U, true); else FillX(U, true, P.bc[0]);
         if (P.bc[1] == 4) FillFixedX(U, false); else FillX(U, false, P.bc[1]);
         if (P.bc[2] == 4) FillFixedY(U, true); else FillY(U, true, P.bc[2]);

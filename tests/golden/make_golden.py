@@ -124,6 +124,9 @@ def cases():
     i3 = P.blast3d(24, 24, 12, bc=(1, 1, 2, 2), bcz=(3, 3), stepmax=15, radius=0.2, center=(0.6, 0.55, 0.3))
     out["inner3d_rk3_g"] = i3.with_(extent=(-1.0, 1.0, -1.0, 1.0, -0.5, 0.5), integrator=3, g=(0.0, -0.1, 0.05),
                                     fix_inner=True)
+    # 3D immersed sphere (src/boundary/immersed.cpp:89-300): an oblique flow with a pressure pulse
+    # upstream, periodic x and z, reflective/outflow y, RK3
+    out["shell3d_rk3"] = P.sphere_flow3d(40, 36, 32, stepmax=15)
     return out
 
 

@@ -29,7 +29,7 @@ def rel_l1(a, b):
                                   "fixedvar52x44_rk2_g", "solid96_rk2", "inner64_rk3_g",
                                   "shellinner72x60_rk3", "blast3d_mixed_rk3_g", "blast3d_z0_rk2",
                                   "blast3d_point_rk2", "fixed3d_mixed_rk3_g", "fixed3d_all4_rk2",
-                                  "inner3d_rk3_g"])
+                                  "inner3d_rk3_g", "shell3d_rk3"])
 def test_driver_golden(tmp_path, name):
     prob, final, t, steps = load_run(name)
     print(f"driver case {name}", flush=True)

@@ -355,6 +355,22 @@ def test_3d_inner_region_vs_oracle(hip, integrator, precision):
     assert np.all(err <= (TOL64 if precision == 64 else 5e-5)), err
 
 
+@pytest.mark.parametrize("integrator,precision", [(3, 64), (1, 64), (2, 32)])
+def test_3d_immersed_sphere_vs_oracle(hip, integrator, precision):
+    """An immersed sphere (src/boundary/immersed.cpp:89-300) of 19 cells' radius in an oblique flow
+    on a 96x88x80 grid (two strips, 80 planes): shell cells over many planes, up to 8 targets each
+    across planes; forward Euler exercises the ping-pong buffers."""
+    from oracle import gko
+    prob = P.sphere_flow3d(96, 88, 80, stepmax=20).with_(integrator=integrator)
+    print(f"3D sphere 96x88x80 integrator={integrator} precision={precision}", flush=True)
+    ref = gko.OracleSim(prob, precision=precision).run_to_end()
+    s = hip.Solver(prob, precision=precision).run_to_end()
+    assert s.step == ref.step == 20
+    err = rel_l1(s.download(), ref.cons().astype(np.float64))
+    print("3D sphere rel L1", err)
+    assert np.all(err <= (TOL64 if precision == 64 else 5e-5)), err
+
+
 def test_3d_sums_follow_the_reference_scheme(hip):
     """Fully periodic 3D is not exactly conservative in the reference: the z faces of the first and
     last rows take y slopes in the z ghost planes, whose edge rows no boundary fills (they keep the
