@@ -25,21 +25,24 @@ __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
 // Neighbour-lane values: lane_below gives lane i the value of lane i-1, lane_above that of lane
-// i+1; the end lane with no source (0, resp. 63) keeps its own value, as __shfl_up/__shfl_down
-// return it.  Doubles move through DPP wavefront shifts (wave_shr:1 / wave_shl:1), two VALU moves
-// instead of two LDS permutes and their wait: the FP64 stage kernels run 2-3% faster.  Floats keep
-// the LDS permute: with DPP moves the FP32 stage kernels ran 7-10% slower.  Every lane of the
-// wave must be active.
+// i+1.  Doubles move through DPP wavefront shifts (wave_shr:1 / wave_shl:1), two VALU moves
+// instead of two LDS permutes and their wait: the FP64 stage kernels run 2-3% faster.  The end
+// lane with no source (0, resp. 63) is not written by the wave shift and receives an unspecified
+// value: each 32-bit half is then one v_mov_b32_dpp into a fresh register, where giving that lane
+// its own value needs a copy into the destination first (30 fewer VALU instructions per row
+// iteration, +1.7% per step).  Lanes 0 and 63 are halo lanes of every strip; no owner lane's
+// result depends on a value they receive (the stencil of owner lane i reaches lanes i-2 .. i+2,
+// owner lanes are 2 .. 61).  Floats keep the LDS permute: with DPP moves the FP32 stage kernels
+// ran 7-10% slower.  Every lane of the wave must be active.
 template <int CTRL>
 __device__ __forceinline__ double dpp_shift(double v) {
     const int lo = __double2loint(v), hi = __double2hiint(v);
-    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false),
-                            __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false));
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, false),
+                            __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, false));
 }
 template <int CTRL>
 __device__ __forceinline__ float dpp_shift(float v) {
-    const int b = __float_as_int(v);
-    return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 __device__ __forceinline__ double lane_below(double v) { return dpp_shift<0x138>(v); }
 __device__ __forceinline__ double lane_above(double v) { return dpp_shift<0x130>(v); }
@@ -851,14 +854,20 @@ __global__ void selftest_kernel(long long n, unsigned long long seed, int emax, 
     unsigned long long local = 0;
     const int emax32 = emax < 60 ? emax : 60;
     {
-        // the DPP lane shifts against the cross-lane shuffles (every lane active)
+        // the DPP lane shifts against the cross-lane shuffles (every lane active) on every lane
+        // that has a source lane (the end lane's value is unspecified)
         const unsigned long long h = mix64(~seed ^ (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x));
         const double v = draw(h, emax, true);
         const float f = drawf((unsigned)h, (unsigned)(h >> 32), emax32);
-        local += dbits(lane_below(v)) != dbits(__shfl_up(v, 1));
-        local += dbits(lane_above(v)) != dbits(__shfl_down(v, 1));
-        local += __float_as_uint(dpp_shift<0x138>(f)) != __float_as_uint(__shfl_up(f, 1));
-        local += __float_as_uint(dpp_shift<0x130>(f)) != __float_as_uint(__shfl_down(f, 1));
+        const int ln = threadIdx.x & (kWave - 1);
+        const double vb = lane_below(v), va = lane_above(v);
+        const float fb = dpp_shift<0x138>(f), fa = dpp_shift<0x130>(f);
+        const double sb = __shfl_up(v, 1), sa = __shfl_down(v, 1);
+        const float tb = __shfl_up(f, 1), ta = __shfl_down(f, 1);
+        local += ln > 0 && dbits(vb) != dbits(sb);
+        local += ln < kWave - 1 && dbits(va) != dbits(sa);
+        local += ln > 0 && __float_as_uint(fb) != __float_as_uint(tb);
+        local += ln < kWave - 1 && __float_as_uint(fa) != __float_as_uint(ta);
     }
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
         const double a = draw(mix64(seed ^ (3ull * k)), emax, true);
@@ -868,6 +877,8 @@ __global__ void selftest_kernel(long long n, unsigned long long seed, int emax, 
         const double r_ieee = __builtin_sqrt(x), r = qsqrt(x);
         local += (__double_as_longlong(q) != __double_as_longlong(q_ieee));
         local += (__double_as_longlong(r) != __double_as_longlong(r_ieee));
+        // the reciprocal form the stencil uses (srcp: the numerator 1 folds into the core)
+        local += (__double_as_longlong(qdiv(1.0, b)) != __double_as_longlong(1.0 / b));
         const unsigned long long h = mix64(seed ^ (3ull * k + 5));
         const float a32 = drawf((unsigned)h, (unsigned)(h >> 32), emax32);
         const float b32 = drawf((unsigned)k, (unsigned)(h >> 40) * 2654435761u, emax32);

@@ -38,8 +38,9 @@ typedef enum {
 } gk_status;
 
 /* Grid: the keys NX, NY, NZ, NGhost, xmin..zmax of the .in file (src/grid.hpp:18-41,
- * src/sim.cpp:16-42).  nz = 1 runs the 1D/2D path, nz > 1 the 3D path (face types 0-3, constant
- * and point gravity; ny >= 2); nghost must be at least 2 (PLM stencil). */
+ * src/sim.cpp:16-42).  nz = 1 runs the 1D/2D path, nz > 1 the 3D path (ny >= 2; every face type,
+ * constant and point gravity, the inner fixed region and the immersed sphere; boundary hooks are
+ * 2D only); nghost must be at least 2 (PLM stencil). */
 typedef struct {
     int nx, ny, nz, nghost;
     double xmin, xmax, ymin, ymax, zmin, zmax;
@@ -171,11 +172,12 @@ gk_status gk_check_finite(gk_ctx* ctx);
 gk_status gk_set_tile(gk_ctx* ctx, int seg_rows);
 gk_status gk_get_tile(gk_ctx* ctx, int* seg_rows, int* strips, int* waves);
 
-/* Device self-check, no counterpart in the reference: the short FP64 quotient and square-root
- * sequences of the stage kernels against the IEEE operators on `count` operand draws with
- * binary exponents in [-emax, emax] (root operands [-2*emax, 2*emax]); *mismatches counts results
- * whose bits differ; also the stage kernels' neighbour-lane moves against the cross-lane shuffles
- * (one draw per launched thread).  Expected 0 for emax <= 500. */
+/* Device self-check, no counterpart in the reference: the short FP64 quotient, reciprocal and
+ * square-root sequences of the stage kernels against the IEEE operators on `count` operand draws
+ * with binary exponents in [-emax, emax] (root operands [-2*emax, 2*emax]), and the FP32 quotient
+ * core; *mismatches counts results whose bits differ; also the stage kernels' neighbour-lane moves
+ * against the cross-lane shuffles on every lane that has a source lane (one draw per launched
+ * thread).  Expected 0 for emax <= 500. */
 gk_status gk_selftest(long long count, unsigned long long seed, int emax, long long* mismatches);
 
 /* ---- Ensembles: many independent grids advanced together (no counterpart in the reference,
