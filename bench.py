@@ -42,9 +42,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+_HOST_CPU = None
+
+
 def host_cpu():
     """The host cores this process may run on: logical CPUs in its affinity mask, the physical
-    cores they belong to, and the CPU model (from /proc/cpuinfo)."""
+    cores they belong to, and the CPU model (from /proc/cpuinfo).  Read once, before the first CPU
+    baseline: the OpenMP runtime binds the calling thread to one core when it starts (proc_bind
+    close), which would shrink the mask seen by later calls."""
+    global _HOST_CPU
+    if _HOST_CPU is None:
+        _HOST_CPU = _read_host_cpu()
+    return dict(_HOST_CPU)
+
+
+def _read_host_cpu():
     cpus = sorted(os.sched_getaffinity(0))
     model, cores, cur = "unknown", set(), {}
     try:
@@ -253,6 +265,11 @@ def bench_rt(args):
                         "bytes_per_launch": bpc * cells, "achieved_GBs": bpc * cells / (avg_ms * 1e-3) / 1e9})
     dom = max(kernels, key=lambda q: q["avg_launch_us"] * q["launches"])
     log("FP32 stage kernels: " + ", ".join(f"{q['avg_launch_us']:.1f} us" for q in kernels))
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_stage_kernels_rt1024x3072_rk2.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch", {}).get(dom["kernel"])
     out = {
         "metric": METRIC, "value": zu[64], "unit": "zone-updates/s", "n_gpus": 1,
         "steps": res[64]["steps"], "warmup": args.warmup, "ms_per_step": res[64]["wall"] * 1e3 / res[64]["steps"],
@@ -264,7 +281,7 @@ def bench_rt(args):
         "mass_drift_fp64": float(res[64]["sums1"][0] / res[64]["sums0"][0] - 1),
         "mass_drift_fp32": float(res[32]["sums1"][0] / res[32]["sums0"][0] - 1),
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom["kernel"],
+                     "frac": dom["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom["kernel"],
                      "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": dom["avg_launch_us"]},
         "fp32_stage_kernels": kernels,
     }
