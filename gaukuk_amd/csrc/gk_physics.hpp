@@ -9,7 +9,8 @@
 // Arithmetic follows the reference's operation order so that the FP64 path differs from the
 // reference only by FMA contraction and reassociation-free rounding (parity target 1e-12
 // relative L1).  Comparisons use std::min/std::max semantics (second operand returned only when
-// strictly better), not fmin/fmax, to keep the reference's signed-zero and NaN behaviour.
+// strictly better), not fmin/fmax, to keep the reference's signed-zero and NaN behaviour, except
+// for maxnum() below, used only where the two agree for every result that matters.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,6 +36,19 @@ template <> __host__ __device__ __forceinline__ float smax<float>(float a, float
     return (a < b) ? b : a;
 #endif
 }
+
+// max(a, b) as the hardware's IEEE maxNum (v_max_f64 / v_max_f32, one instruction instead of a
+// compare and two selects).  It equals smax(a, b) for all operands that are not NaN, except that
+// the sign of a zero result may differ; used where a NaN operand makes the result NaN through
+// another path anyway, and no zero sign reaches the result.  The host build uses smax.
+template <class R> __host__ __device__ __forceinline__ R maxnum(R a, R b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fmax(a, b);
+#else
+    return smax(a, b);
+#endif
+}
+template <> __host__ __device__ __forceinline__ float maxnum<float>(float a, float b) { return smax(a, b); }
 
 template <class R> __host__ __device__ __forceinline__ R gsqrt(R x);
 template <> __host__ __device__ __forceinline__ double gsqrt<double>(double x) { return __builtin_sqrt(x); }
@@ -187,7 +201,8 @@ struct Adiabatic {
         w[3] = u[3] * inv;
         R ek = R(0.5) * inv * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
         R p = gm1 * (u[4] - ek);
-        w[4] = (p > R(GK_PRESSURE_FLOOR)) ? p : R(GK_PRESSURE_FLOOR);
+        // (p > floor) ? p : floor, which is maxNum(p, floor) for every p, NaN and -0 included
+        w[4] = maxnum(p, R(GK_PRESSURE_FLOOR));
     }
 
     // One cell's max_d |v_d| + c_s with CalCmax's own kinetic-energy form: adiabatic.cpp:115-160.
@@ -258,12 +273,14 @@ struct Hllc {
         const R denr = wr[0], vxr = wr[1], vyr = wr[2], vzr = wr[3], prer = wr[4];
         const R al = eos.sound_speed(denl, prel);
         const R ar = eos.sound_speed(denr, prer);
+        // maxnum: pv and tmp are NaN only for a NaN state value, which also makes S* below NaN and
+        // with it every flux component; the sign of a zero pv only reaches tmp = pv/p - 1 = -1
         R pv = R(0.5) * (prel + prer) - R(0.125) * (vxr - vxl) * (denl + denr) * (al + ar);
-        pv = smax(pv, R(0));
+        pv = maxnum(pv, R(0));
         const R qk = R(0.5) * (R(1) + eos.gmRec);
-        R tmp = smax(sdiv(pv, prel) - R(1), R(0));
+        R tmp = maxnum(sdiv(pv, prel) - R(1), R(0));
         const R ql = ssqrt(R(1) + qk * tmp);
-        tmp = smax(sdiv(pv, prer) - R(1), R(0));
+        tmp = maxnum(sdiv(pv, prer) - R(1), R(0));
         const R qr = ssqrt(R(1) + qk * tmp);
         const R sl = vxl - al * ql;
         const R sr = vxr + ar * qr;
