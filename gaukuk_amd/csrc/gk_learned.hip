@@ -140,19 +140,7 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const TrainArgs a) {
         // forward, keeping the pre-activations for the ReLU masks
         f32x4 z1[4], z2[4];
         bf16x4 h1[4], h2[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) { z1[t] = mlp::mfma(w.w1[t], x, w.b1[t]); h1[t] = mlp::pack4(mlp::relu(z1[t])); }
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            f32x4 z = w.b2[t];
-#pragma unroll
-            for (int kb = 0; kb < 4; kb++) z = mlp::mfma(w.w2[t][kb], h1[kb], z);
-            z2[t] = z;
-            h2[t] = mlp::pack4(mlp::relu(z));
-        }
-        f32x4 y = w.b3;
-#pragma unroll
-        for (int kb = 0; kb < 4; kb++) y = mlp::mfma(w.w3[kb], h2[kb], y);
+        const f32x4 y = mlp::forward_tile_keep(w, x, z1, z2, h1, h2);
         // output error (lanes 0..15 hold the 4 outputs of sample r)
         f32x4 dy = zero;
         if (g == 0) {
@@ -605,11 +593,17 @@ __global__ __launch_bounds__(256) void face_targets_kernel(const double* facc, i
 }
 
 // ---------------------------------------------------------------------------------------------
-// The corrected FP32 stage: one workgroup per 16 x 16 tile of cells.
+// The corrected FP32 stage: one 512-thread workgroup per 16 x 8 tile of cells.
 //   1. the tile and a 2-cell halo -> LDS (conserved after the boundary rules, and primitives)
-//   2. PLM + HLLC at the tile's 272 x faces and 272 y faces -> LDS
-//   3. the MLP on 34 tiles of 16 faces (BF16 MFMA), sensor and positivity tests, correction
+//   2. PLM + HLLC at the tile's 136 x faces and 144 y faces -> LDS, one face per thread
+//   3. the MLP on 18 tiles of 16 faces (BF16 MFMA), sensor and positivity tests, correction
 //   4. flux divergence, RK combine, floors, store, max wave speed
+// A tile this size needs 53 KB of LDS and 128 registers per lane, so two workgroups (16 waves)
+// share a CU.  (A 16 x 16 tile needed 88 KB, one workgroup per CU, and 544 faces for 512 threads:
+// two rounds of every face phase.  At 256^2 the two shapes measured the same, 17-18 us per launch:
+// the phases are latency chains separated by barriers, not issue-bound.)  Every face is solved
+// from the same inputs whatever tile holds it, and the network treats each face (one MFMA column)
+// independently, so results do not depend on the tile shape.
 // Weights image: BF16 matrices and FP32 biases/scales in the LDS layout, rebuilt when the
 // network's parameters or scales change.  kImageCopies replicas (one per block of this kernel):
 // every workgroup of a stage reads the image at the same moment, and one copy read by all of
@@ -626,30 +620,37 @@ struct NetArgs {
     int enabled;
 };
 
-constexpr int kLT = 16;                    // tile edge (cells)
-constexpr int kLE = kLT + 2 * kHalo;       // tile edge with halo
-constexpr int kLFx = (kLT + 1) * kLT;      // x faces per tile (y faces: the same count)
-constexpr int kLFaces = 2 * kLFx;          // 544 = 34 tiles of 16
+constexpr int kLTX = 16, kLTY = 8;         // tile width and height (cells)
+constexpr int kLEX = kLTX + 2 * kHalo;     // tile width and height with halo
+constexpr int kLEY = kLTY + 2 * kHalo;
+constexpr int kLFx = (kLTX + 1) * kLTY;    // x faces per tile (136)
+constexpr int kLFy = kLTX * (kLTY + 1);    // y faces per tile (144)
+constexpr int kLFaces = kLFx + kLFy;       // 280
+constexpr int kLFacesPad = (kLFaces + 15) / 16 * 16;   // whole MLP tiles of 16 faces (288)
+constexpr int kLMinSide = 16;              // smallest grid side the corrected stage takes
 
-constexpr int kLThreads = 512;             // 8 waves per tile: two per SIMD
+constexpr int kLThreads = 512;             // 8 waves per tile: one face per thread in every face phase
 constexpr int kLWaves = kLThreads / kWave;
 
+// Two workgroups (16 waves) per CU: the register budget is 128 per lane, VGPRs and AGPRs together
+// (without the bound the compiler assumes fewer waves and spends more on MFMA accumulators).
 template <bool THREE, bool LAST>
-__global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
+__global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
     DevClock* clk = a.clock;
     if (!clk->active) return;
     __shared__ mlp::Smem s;
-    __shared__ float sw[5][kLE][kLE + 1];
-    __shared__ float su[5][kLE][kLE + 1];
+    __shared__ float sw[5][kLEY][kLEX + 1];
+    __shared__ float su[5][kLEY][kLEX + 1];
     __shared__ float sf[5][kLFaces];
     __shared__ unsigned long long scount[kLWaves][3];
-    __shared__ unsigned short sx[kLFaces * mlp::kP16];   // scaled BF16 network inputs per face
-    __shared__ float smeta[kLFaces][8];                  // bound D[4], rho_bar, c_bar, vt_bar, sensor
-    __shared__ float sy[kLFaces][4];                     // network outputs
+    __shared__ unsigned short sx[kLFacesPad * mlp::kP16];   // scaled BF16 network inputs per face
+    __shared__ float smeta[kLFaces][8];                     // bound D[4], rho_bar, c_bar, vt_bar, sensor
+    __shared__ float sy[kLFacesPad][4];                     // network outputs (rows past kLFaces unused)
     __shared__ double swmax[kLWaves];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int tiles_x = (a.nx + kLT - 1) / kLT;
-    const int i0 = (blockIdx.x % tiles_x) * kLT, j0 = (blockIdx.x / tiles_x) * kLT;
+    const int tiles_x = (a.nx + kLTX - 1) / kLTX;
+    const int i0 = (blockIdx.x % tiles_x) * kLTX, j0 = (blockIdx.x / tiles_x) * kLTY;
     const double dt = clk->dt;
     const float dtdx = float(dt * a.dxRec), dtdy = float(dt * a.dyRec);
     const float* in = a.in;
@@ -657,12 +658,12 @@ __global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArg
     if (a.pingpong && clk->cur) { in = a.out; out = const_cast<float*>(a.in); }
     const float gamma = a.eos.gamma;
     // Every global load of the kernel is issued here, before any result is used, so the
-    // workgroup waits out one memory latency: the tile and halo cells (400, one per thread),
+    // workgroup waits out one memory latency: the tile and halo cells (240, one per thread),
     // the combine operand of this thread's cell, and the weights image.
-    constexpr int kCells = kLE * kLE;
+    constexpr int kCells = kLEX * kLEY;
     static_assert(kCells <= kLThreads, "one cell per thread");
     const int kc = min(tid, kCells - 1);
-    const int ly = kc / kLE, lx = kc - ly * kLE;
+    const int ly = kc / kLEX, lx = kc - ly * kLEX;
     int kx, ky;
     const int si = bc_remap(min(max(i0 + lx - kHalo, -2), a.nx + 1), a.nx, a.bc[0], a.bc[1], kx);
     const int sj = bc_remap(min(max(j0 + ly - kHalo, -2), a.ny + 1), a.ny, a.bc[2], a.bc[3], ky);
@@ -670,9 +671,9 @@ __global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArg
     float u[5];
 #pragma unroll
     for (int n = 0; n < 5; n++) u[n] = in[oc + n * a.plane];
-    const int cx = tid % kLT, cy = min(tid / kLT, kLT - 1);
+    const int cx = tid % kLTX, cy = min(tid / kLTX, kLTY - 1);
     const int i = i0 + cx, j = j0 + cy;
-    const bool mine = tid < kLT * kLT && i < a.nx && j < a.ny;
+    const bool mine = tid < kLTX * kLTY && i < a.nx && j < a.ny;
     const long long o = (long long)min(j, a.ny - 1) * a.pitch + min(i, a.nx - 1);
     float bse[5];
 #pragma unroll
@@ -692,10 +693,11 @@ __global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArg
     if (net.enabled) wimg.store(s, tid);
     __syncthreads();
 
-    // face f: x faces fx = 0..16 of row fy (between cells fx-1 and fx), then y faces
+    // face f: x faces fx = 0..16 of row fy (between cells fx-1 and fx), then y faces fy = 0..8 of
+    // column fx (between rows fy-1 and fy)
     auto face_cell = [&](int f, int k, int& cy, int& cx) {  // cell k = 0..3 across face f (LDS coords)
-        if (f < kLFx) { const int fy = f / (kLT + 1), fx = f - fy * (kLT + 1); cy = fy + kHalo; cx = fx + k; }
-        else { const int f2 = f - kLFx, fy = f2 / kLT, fx = f2 - fy * kLT; cy = fy + k; cx = fx + kHalo; }
+        if (f < kLFx) { const int fy = f / (kLTX + 1), fx = f - fy * (kLTX + 1); cy = fy + kHalo; cx = fx + k; }
+        else { const int f2 = f - kLFx, fy = f2 / kLTX, fx = f2 - fy * kLTX; cy = fy + k; cx = fx + kHalo; }
     };
     for (int f = tid; f < kLFaces; f += kLThreads) {
         float w[4][5], F[5];
@@ -799,8 +801,8 @@ __global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArg
                 for (int n = 0; n < 5; n++) sf[n][f] += d5[n];
             }
             // counts: each grid face once (faces on a tile's high edge belong to the next tile)
-            const int fl = yface ? (f - kLFx) / kLT : f % (kLT + 1);
-            const bool own = fl < kLT && (yface ? j0 + fl < a.ny : i0 + fl < a.nx);
+            const int fl = yface ? (f - kLFx) / kLTX : f % (kLTX + 1);
+            const bool own = fl < (yface ? kLTY : kLTX) && (yface ? j0 + fl < a.ny : i0 + fl < a.nx);
             if (own) {
                 if (fired) n_sensor++;
                 else if (ok) n_corr++;
@@ -829,11 +831,11 @@ __global__ __launch_bounds__(kLThreads) void learned_stage_kernel(const StageArg
 
     float cacc = 0.f;
     if (mine) {
-        const int fxl = cy * (kLT + 1) + cx, fyl = kLFx + cy * kLT + cx;
+        const int fxl = cy * (kLTX + 1) + cx, fyl = kLFx + cy * kLTX + cx;
         float res[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-            const float dF = (sf[n][fxl + 1] - sf[n][fxl]) * dtdx + (sf[n][fyl + kLT] - sf[n][fyl]) * dtdy;
+            const float dF = (sf[n][fxl + 1] - sf[n][fxl]) * dtdx + (sf[n][fyl + kLTX] - sf[n][fyl]) * dtdy;
             const float ut = su[n][cy + kHalo][cx + kHalo];
             if (THREE) res[n] = a.c1 * bse[n] + a.c2 * ut - a.c3 * dF;
             else res[n] = a.c1 * ut - a.c2 * dF;
@@ -987,7 +989,7 @@ static void record_launch(gk_ctx* c, size_t q) {
                            EosPolicy<double>::make(c->phys.gamma), c->clock, c->rec_w[q], c->acc);
 }
 
-static int learned_tiles(const gk_ctx* c) { return ((c->nx + kLT - 1) / kLT) * ((c->ny + kLT - 1) / kLT); }
+static int learned_tiles(const gk_ctx* c) { return ((c->nx + kLTX - 1) / kLTX) * ((c->ny + kLTY - 1) / kLTY); }
 
 static bool learned_launch(gk_ctx* c, const StageSpec& s) {
     if (!c->net || c->prec != 32) return false;
@@ -1387,7 +1389,7 @@ gk_status gk_set_net(gk_ctx* c, gk_net* n) {
     if (!c) return GK_EINVAL;
     if (!n) { c->net = nullptr; return GK_OK; }
     if (const char* m = net_conflict(c)) return fail(c, GK_EINVAL, m);
-    if (c->nx < kLT || c->ny < kLT) return fail(c, GK_EINVAL, "the corrected stage needs NX, NY >= 16");
+    if (c->nx < kLMinSide || c->ny < kLMinSide) return fail(c, GK_EINVAL, "the corrected stage needs NX, NY >= 16");
     if (!c->lstats) {
         const size_t bytes = sizeof(unsigned long long) * 3 * (size_t)learned_tiles(c);
         GK_TRY_HIP(c, hipMalloc(&c->lstats, bytes));

@@ -1,12 +1,17 @@
 // gk_mlp.hpp -- the flux-correction MLP (16 -> 64 -> 64 -> 4, ReLU) on CDNA4 matrix cores.
 //
-// No counterpart in the reference.  Every product is v_mfma_f32_16x16x16_bf16: BF16 operands,
-// FP32 accumulation.  The network runs in transposed form, H^T = W X^T, 16 samples per tile:
+// No counterpart in the reference.  BF16 operands, FP32 accumulation on the matrix cores.  The
+// network runs in transposed form, H^T = W X^T, 16 samples per tile.  With v_mfma_f32_16x16x16_bf16:
 //   A operand (16 x 16, rows x k): lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
 //   B operand (16 x 16, k x cols): lane l holds B[4 (l / 16) + i][l % 16]
 //   D result  (16 x 16):           lane l holds D[4 (l / 16) + i][l % 16]
 // so the D tile of one layer (rows = hidden units, columns = samples) is already the B operand of
-// the next: no data moves between layers.  Weights are the A operands, read once per wave.  The
+// the next: no data moves between layers.  The contractions over 64 hidden units (layers 2 and 3
+// of the forward pass) use gfx950's v_mfma_f32_16x16x32_bf16, half the instructions: a lane's 8
+// operand values (slot i of lane group g pairs A with B) are the 4 values it holds of two
+// consecutive 16-unit blocks 2c and 2c+1, so chunk c sums units 16(2c) + 4g + i and
+// 16(2c+1) + 4g + i over g and i -- the same 32 units in another order on both operands, built
+// from registers each lane already holds.  Weights are the A operands, read once per wave.  The
 // weight gradients contract over samples, which needs each tile transposed (through LDS).
 //
 // Parameter vector (FP32 master copy, 5508 values): W1[64][16], b1[64], W2[64][64], b2[64],
@@ -27,6 +32,8 @@ constexpr int kP16 = 20;
 constexpr int kP64 = 68;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16, round to nearest, ties to even (finite inputs).
@@ -50,6 +57,14 @@ __device__ __forceinline__ f32x4 relu(f32x4 v) {
 }
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// K = 32: the operands of two consecutive 16-wide blocks, concatenated per lane
+__device__ __forceinline__ bf16x8 cat(bf16x4 lo, bf16x4 hi) {
+    return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b), c,
+                                                   0, 0, 0);
 }
 __device__ __forceinline__ bf16x4 lds4(const unsigned short* p) {
     return *reinterpret_cast<const bf16x4*>(p);
@@ -124,11 +139,11 @@ __device__ __forceinline__ void stage_t(SmemT& s, const float* p, int tid, int n
     }
 }
 
-// A operands and biases of one wave's lane, loaded once.
+// A operands and biases of one wave's lane, loaded once; layers 2 and 3 in K = 32 chunks c.
 struct Regs {
     bf16x4 w1[4];      // [rt]: W1[16 rt + r][4 g .. 4 g + 3]
-    bf16x4 w2[4][4];   // [ro][kb]: W2[16 ro + r][16 kb + 4 g .. + 3]
-    bf16x4 w3[4];      // [kb]: r < 4 ? W3[r][16 kb + 4 g .. + 3] : 0
+    bf16x8 w2[4][2];   // [ro][c]: W2[16 ro + r][32 c + 4 g .. + 3], then W2[16 ro + r][32 c + 16 + 4 g .. + 3]
+    bf16x8 w3[2];      // [c]: r < 4 ? (W3[r][32 c + 4 g .. + 3], W3[r][32 c + 16 + 4 g .. + 3]) : 0
     f32x4 b1[4], b2[4], b3;
 };
 
@@ -138,15 +153,20 @@ __device__ __forceinline__ void load_regs(Regs& w, const Smem& s, int lane) {
     for (int t = 0; t < 4; t++) {
         w.w1[t] = lds4(&s.w1[(16 * t + r) * kP16 + 4 * g]);
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) w.w2[t][kb] = lds4(&s.w2[(16 * t + r) * kP64 + 16 * kb + 4 * g]);
-        const bf16x4 z = {0, 0, 0, 0};
-        w.w3[t] = r < kOut ? lds4(&s.w3[r * kP64 + 16 * t + 4 * g]) : z;
+        for (int c = 0; c < 2; c++)
+            w.w2[t][c] = cat(lds4(&s.w2[(16 * t + r) * kP64 + 32 * c + 4 * g]),
+                             lds4(&s.w2[(16 * t + r) * kP64 + 32 * c + 16 + 4 * g]));
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             w.b1[t][i] = s.b1[16 * t + 4 * g + i];
             w.b2[t][i] = s.b2[16 * t + 4 * g + i];
         }
     }
+    const bf16x4 z = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+        w.w3[c] = r < kOut ? cat(lds4(&s.w3[r * kP64 + 32 * c + 4 * g]), lds4(&s.w3[r * kP64 + 32 * c + 16 + 4 * g]))
+                           : cat(z, z);
 #pragma unroll
     for (int i = 0; i < 4; i++) w.b3[i] = g == 0 ? s.b3[i] : 0.f;
 }
@@ -154,21 +174,29 @@ __device__ __forceinline__ void load_regs(Regs& w, const Smem& s, int lane) {
 // Forward pass of one 16-sample tile.  x: lane (g, s) holds the scaled inputs 4g..4g+3 of sample
 // s as BF16.  Returns the D tile of the output layer: lanes 0..15 hold outputs 0..3 of sample
 // `lane`; other lanes hold zeros.
-__device__ __forceinline__ f32x4 forward_tile(const Regs& w, bf16x4 x) {
-    bf16x4 h1[4], h2[4];
+// The same with the pre-activations and activations kept (training needs the ReLU masks and the
+// operands of the weight gradients).
+__device__ __forceinline__ f32x4 forward_tile_keep(const Regs& w, bf16x4 x, f32x4 z1[4], f32x4 z2[4], bf16x4 h1[4],
+                                                   bf16x4 h2[4]) {
 #pragma unroll
-    for (int t = 0; t < 4; t++) h1[t] = pack4(relu(mfma(w.w1[t], x, w.b1[t])));
+    for (int t = 0; t < 4; t++) { z1[t] = mfma(w.w1[t], x, w.b1[t]); h1[t] = pack4(relu(z1[t])); }
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         f32x4 z = w.b2[t];
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) z = mfma(w.w2[t][kb], h1[kb], z);
+        for (int c = 0; c < 2; c++) z = mfma32(w.w2[t][c], cat(h1[2 * c], h1[2 * c + 1]), z);
+        z2[t] = z;
         h2[t] = pack4(relu(z));
     }
     f32x4 y = w.b3;
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++) y = mfma(w.w3[kb], h2[kb], y);
+    for (int c = 0; c < 2; c++) y = mfma32(w.w3[c], cat(h2[2 * c], h2[2 * c + 1]), y);
     return y;
+}
+__device__ __forceinline__ f32x4 forward_tile(const Regs& w, bf16x4 x) {
+    f32x4 z1[4], z2[4];
+    bf16x4 h1[4], h2[4];
+    return forward_tile_keep(w, x, z1, z2, h1, h2);
 }
 
 }  // namespace mlp
