@@ -1209,6 +1209,9 @@ gk_status gk_set_bd_hook(gk_ctx* c, int face_mask, int band, gk_bd_hook hook, vo
         return fail(c, GK_EINVAL, "gk_set_bd_hook: band must lie in [1, min(nx, ny)]");
     if (c->npoint) return fail(c, GK_EINVAL, "boundary hooks do not take point gravity");
     if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
+    // a hooked face became a type-4 face; without its hook it would keep its last ghost layers
+    if (c->hook && (c->hook_mask & ~face_mask))
+        return fail(c, GK_EINVAL, "gk_set_bd_hook: a later call must keep every face an earlier call hooked");
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
     for (int f = 0; f < 4; f++) {
         if (!((face_mask >> f) & 1)) continue;

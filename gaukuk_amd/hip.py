@@ -314,10 +314,11 @@ class Solver:
         [5][NZ+2g][NY+2g][NX+2g] in the solver's precision (NaN except the mirrored bands of
         `band` active layers next to those faces, which the library writes before each call)."""
         dt = np.float64 if self.precision == 64 else np.float32
-        self._mirror = np.full(self._ref_shape(), np.nan, dt)
-        self._hook_fn = BD_HOOK(lambda user, ptr: fn(self._mirror))
-        self._check(self.L.gk_set_bd_hook(self.h, face_mask, band, self._hook_fn, None,
-                                          self._mirror.ctypes.data))
+        mirror = np.full(self._ref_shape(), np.nan, dt)
+        hook = BD_HOOK(lambda user, ptr: fn(mirror))
+        self._check(self.L.gk_set_bd_hook(self.h, face_mask, band, hook, None, mirror.ctypes.data))
+        # the library holds both from now on (a failed call leaves the previous ones in place)
+        self._mirror, self._hook_fn = mirror, hook
 
     def update_bd(self):
         """Boundary::UpdateBD's in-place part on the current state (immersed shell, inner region),

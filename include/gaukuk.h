@@ -101,7 +101,8 @@ gk_status gk_set_immersed_shell(gk_ctx* ctx, double x, double y, double z, doubl
  * fail the step with GK_EINVAL.  Stepping returns to the host once per stage; gk_download_cons
  * fills the hooked faces' ghost cells by calling the hook on the current state.  Fails with
  * GK_EINVAL for 3D grids, a NULL hook or mirror, band outside [1, min(nx, ny)], y faces of a
- * one-row grid, point gravity or an attached network.  A later call replaces the hook. */
+ * one-row grid, point gravity or an attached network.  A later call replaces the hook; its mask
+ * must include every face an earlier call hooked (GK_EINVAL otherwise). */
 typedef void (*gk_bd_hook)(void* user, void* mirror);
 gk_status gk_set_bd_hook(gk_ctx* ctx, int face_mask, int band, gk_bd_hook hook, void* user, void* mirror);
 

@@ -107,3 +107,11 @@ def test_hook_rejects_unsupported(hip):
     s.set_bd_hook(1, 2, copy_outflow(prob.ng, 1))
     with pytest.raises(hip.GaukukError, match="point gravity"):
         s._check(s.L.gk_set_point_gravity(s.h, 1.0, 0.5, 0.5, 0.0, 0.0, 0.0, 0.0, 0.1))
+    # a later call may add faces but not drop a hooked one (it would keep stale ghost layers);
+    # the refused call leaves the first hook working
+    with pytest.raises(hip.GaukukError, match="keep every face"):
+        s.set_bd_hook(2, 2, copy_outflow(prob.ng, 2))
+    s.run_to_end()
+    ref = hip.Solver(prob).run_to_end()
+    assert s.step == ref.step == 3
+    assert np.array_equal(s.download(), ref.download())
