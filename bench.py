@@ -328,14 +328,36 @@ def bench_3d(args):
     print(json.dumps(out), flush=True)
 
 
+def time_to_l1(r, key, n: int = 256):
+    """Wall-time ratio at equal accuracy: for each output time and conserved variable (mass, m_x,
+    m_y, E), the resolution N at which plain FP64 would reach the corrected solver's L1 (at the
+    convergence order plain FP64 shows between n^2 and (2n)^2), and the wall time of plain FP64 at
+    N (cost ~ N^3: cells x steps, from its measured n^2 run) over the corrected solver's."""
+    out = []
+    for k in range(len(r[key]["l1"])):
+        row = []
+        for q in range(4):
+            e1, e2, el = r[f"fp64_{n}"]["l1"][k][q], r[f"fp64_{2 * n}"]["l1"][k][q], r[key]["l1"][k][q]
+            if not (e1 > e2 > 0 and el > 0):
+                row.append(None)
+                continue
+            order = np.log2(e1 / e2)
+            n_eq = n * (e1 / el) ** (1.0 / order)
+            row.append({"order": float(order), "fp64_n_equivalent": float(n_eq),
+                        "wall_ratio": float((n_eq / n) ** 3 * r[f"fp64_{n}"]["wall_s"] / r[key]["wall_s"])})
+        out.append(row)
+    return out
+
+
 def bench_learned(args):
-    """Learned flux correction on coarse KH: the corrected FP32 256^2 solver (the network trained on
-    early frames, gaukuk_amd/data/kh256_flux_mlp_early.npz, kappa 1) against plain FP64 at 256^2
-    and 512^2 and plain FP32 at 256^2, from the held-out seed 1000 to t = 0.025 and 0.05: wall
-    time, steps and relative L1 against the restricted 1024^2 FP64 run.  value = zone-updates/s of
-    the corrected solver; roofline = the corrected stage kernel's MLP flops on MFMA."""
+    """Learned flux correction on coarse KH: the corrected FP32 256^2 solver (the early-frame network
+    fine-tuned a posteriori on whole-run errors, gaukuk_amd/data/kh256_flux_mlp_es.npz, kappa 1)
+    against plain FP64 at 256^2 and 512^2 and plain FP32 at 256^2, from the held-out seed 1000 to
+    t = 0.025, 0.05 and 0.1: wall time, steps, relative L1 against the restricted 1024^2 FP64 run,
+    and the time-to-equal-L1 estimate.  value = zone-updates/s of the corrected solver; roofline =
+    the corrected stage kernel's MLP flops on MFMA."""
     from gaukuk_amd import hip, learned
-    net = hip.Net.load(os.path.join(ROOT, "gaukuk_amd", "data", "kh256_flux_mlp_early.npz"))
+    net = hip.Net.load(os.path.join(ROOT, "gaukuk_amd", "data", "kh256_flux_mlp_es.npz"))
     n = 256
     prob = learned.kh(1024, 1000)
     coarse = prob.with_(nx=n, ny=n, cons=learned.restrict(prob.cons, 1024 // n))
@@ -355,28 +377,32 @@ def bench_learned(args):
     dom = int(np.argmax([kms[k] for k in range(3)]))
     log(f"corrected stage kernel per launch: {[round(a * 1e3, 1) for a in avg_ms]} us; "
         f"clock kernels {kms[3] / max(cnt[3], 1) * 1e3:.1f} us")
-    log("accuracy and time to t = 0.025, 0.05 against the 1024^2 FP64 run")
+    times = (0.025, 0.05, 0.1)
+    log("accuracy and time to t = 0.025, 0.05, 0.1 against the 1024^2 FP64 run")
     kappa = float(net.kappa)
-    ev = learned.evaluate(net, seeds=(1000,), times=(0.025, 0.05), kappas=(kappa,), log=log)
+    ev = learned.evaluate(net, seeds=(1000,), times=times, kappas=(kappa,), log=log)
     r = ev["seeds"]["1000"]
     key = f"learned_fp32_k{kappa:g}_{n}"
     gain = {f"t{t}": [1 - c / p for c, p in zip(r[key]["l1"][k], r["fp32_256"]["l1"][k])]
-            for k, t in enumerate((0.025, 0.05))}
+            for k, t in enumerate(times)}
+    ttl = time_to_l1(r, key, n)
     out = {
         "metric": METRIC, "value": n * n * K / wall, "unit": "zone-updates/s", "n_gpus": 1,
         "steps": K, "warmup": args.warmup, "ms_per_step": wall * 1e3 / K, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
         "config": {"workload": "Kelvin-Helmholtz 256x256 FP32 with the learned flux correction (MLP 16-64-64-4, "
                                "BF16 MFMA), PLM+HLLC+RK3, CFL 0.4, periodic, seeded noise on 64^2 cells, seed 1000",
-                   "nx": n, "ny": n, "kappa": kappa},
+                   "nx": n, "ny": n, "kappa": kappa, "network": "kh256_flux_mlp_es.npz"},
         "roofline": {"bound": "mfma", "achieved": flops / (avg_ms[dom] * 1e-3) / 1e12, "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": flops / (avg_ms[dom] * 1e-3) / 1e12 / 2500.0, "traffic": None,
                      "kernel": f"learned_stage_kernel (RK3 stage {dom + 1})", "flops_per_launch": flops,
                      "avg_launch_us": avg_ms[dom] * 1e3},
-        "time_to_t": {k: {"wall_s": v["wall_s"], "steps": v["steps"], "l1_t0.025": v["l1"][0], "l1_t0.05": v["l1"][1]}
+        "time_to_t": {k: {"wall_s": v["wall_s"], "steps": v["steps"],
+                          **{f"l1_t{t}": v["l1"][q] for q, t in enumerate(times)}}
                       for k, v in r.items() if k != "fine"},
         "fine_reference": r["fine"],
         "gain_vs_plain_fp32_256": gain,
+        "time_to_equal_l1_fp64_over_learned": {f"t{t}": ttl[q] for q, t in enumerate(times)},
         "wall_ratio_fp64_512_over_learned": r["fp64_512"]["wall_s"] / r[key]["wall_s"],
         "wall_ratio_fp64_256_over_learned": r["fp64_256"]["wall_s"] / r[key]["wall_s"],
         "fallback_faces": r[key].get("faces"),

@@ -11,10 +11,15 @@ The pipeline, all on the GPU through the C ABI (include/gaukuk.h):
    from the device data set (``gk_net_train``: BF16 MFMA forward/backward, FP32 master weights).
 3. ``evaluate``: a held-out seed; L1 error of each coarse solver against the 1024^2 FP64 run
    restricted to the coarse grid, and each solver's wall time to t_end.
+4. ``finetune``: a posteriori, on the error of whole corrected runs rather than of one step:
+   evolution strategies on the run-level objective (below), the corrected GPU solver being the
+   only model evaluated (no adjoint of the stage is needed).
 
 Usage (writes the trained network as .npz):
     python -m gaukuk_amd.learned train --seeds 64 --out gaukuk_amd/data/kh256_flux_mlp.npz
     python -m gaukuk_amd.learned evaluate --net gaukuk_amd/data/kh256_flux_mlp.npz
+    python -m gaukuk_amd.learned finetune --net gaukuk_amd/data/kh256_flux_mlp_early.npz \
+        --out gaukuk_amd/data/kh256_flux_mlp_es.npz
 """
 from __future__ import annotations
 
@@ -173,6 +178,76 @@ def evaluate(net, seeds=(1000,), times=(0.3, 0.5), kappas=(0.5,), n_fine: int = 
     return res
 
 
+# Parameter blocks of the network (W1, b1, W2, b2, W3, b3): each weight block and its layer's bias
+# are perturbed on the scale of that layer's RMS weight.
+_BLOCKS = ((0, 1024, 1088), (1088, 5184, 5248), (5248, 5504, 5508))
+
+
+def finetune(net, seeds=(0, 1, 2, 3), times=(0.025, 0.05, 0.1), iters: int = 120, pairs: int = 12,
+             sigma: float = 0.02, lr: float = 0.01, seed: int = 0, n_fine: int = 1024, n_coarse: int = 256,
+             log=print) -> list:
+    """A posteriori fine-tuning by evolution strategies (antithetic Gaussian perturbations, rank
+    utilities, Adam).  Objective: the corrected FP32 solver's relative L1 (mass, m_x, m_y, E) against
+    the restricted n_fine FP64 run, divided by the plain FP32 solver's, averaged over variables,
+    output times and seeds (1 = no gain).  A run that does not reach the last time (its dt collapsed
+    under a step cap of 3x the plain run's steps) scores 10 per variable.  Keeps the best parameters
+    seen on the unperturbed objective and leaves them in `net`.  Returns [[iteration, objective]]."""
+    from . import hip
+    f = n_fine // n_coarse
+    refs, probs, plain = {}, {}, {}
+    for s in seeds:
+        fp = kh(n_fine, s)
+        us, _, _, _, _ = timed_run(fp, 64, times, repeats=1)
+        refs[s] = [restrict(u, f) for u in us]
+        probs[s] = fp.with_(nx=n_coarse, ny=n_coarse, cons=restrict(fp.cons, f))
+        pu, _, steps, _, _ = timed_run(probs[s], 32, times, repeats=1)
+        plain[s] = [np.array(rel_l1(u, r)) for u, r in zip(pu, refs[s])]
+        probs[s] = probs[s].with_(stepmax=3 * steps)
+
+    def objective(theta):
+        net.set_params(theta.astype(np.float32))
+        tot = 0.0
+        for s in seeds:
+            sv = hip.Solver(probs[s], precision=32)
+            sv.set_net(net)
+            for k, t in enumerate(times):
+                if sv.time()[3]:
+                    sv.advance(t - sv.t)
+                e = np.array(rel_l1(sv.download(), refs[s][k])) if abs(sv.t - t) < 1e-9 else np.full(4, 10.0)
+                tot += float(np.mean(e / plain[s][k]))
+            sv.close()
+        return tot / (len(seeds) * len(times))
+
+    theta = net.params().astype(np.float64)
+    scale = np.empty_like(theta)
+    for a, b, c in _BLOCKS:
+        scale[a:c] = np.sqrt(np.mean(theta[a:b] ** 2)) + 1e-8
+    rng = np.random.default_rng(seed)
+    m, v = np.zeros_like(theta), np.zeros_like(theta)
+    best = (objective(theta), theta.copy())
+    hist = [[0, best[0]]]
+    log(f"finetune: start objective {best[0]:.4f}")
+    for it in range(1, iters + 1):
+        eps = rng.standard_normal((pairs, theta.size))
+        fs = np.array([objective(theta + sigma * scale * e) for e in eps] +
+                      [objective(theta - sigma * scale * e) for e in eps])
+        ranks = np.empty(2 * pairs)
+        ranks[np.argsort(fs)] = np.arange(2 * pairs)
+        util = 0.5 - ranks / (2 * pairs - 1)           # the lowest objective gets +0.5
+        g = -((util[:pairs] - util[pairs:])[:, None] * eps).sum(0) / pairs
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        theta = theta - lr * sigma * scale * (m / (1 - 0.9 ** it)) / (np.sqrt(v / (1 - 0.999 ** it)) + 1e-8)
+        if it % 5 == 0 or it == iters:
+            fo = objective(theta)
+            hist.append([it, fo])
+            if fo < best[0]:
+                best = (fo, theta.copy())
+            log(f"finetune iter {it}: objective {fo:.4f}, best {best[0]:.4f}")
+    net.set_params(best[1].astype(np.float32))
+    return hist
+
+
 def one_step_error(net, seed: int = 1000, t0: float = 0.3, n_fine: int = 1024, n_coarse: int = 256) -> dict:
     """Local (one coarse step) error of the plain and the corrected FP32 coarse solvers: from the
     restricted state of the n_fine FP64 run at t0, one coarse step of length D (both solvers take
@@ -226,13 +301,40 @@ def main(argv=None) -> int:
     ev.add_argument("--eval-times", default="0.3,0.5")
     ev.add_argument("--kappas", default="0.25,0.5,1.0")
     ev.add_argument("--report", default=None)
+    ft = sub.add_parser("finetune")
+    ft.add_argument("--net", default=os.path.join(HERE, "data", "kh256_flux_mlp_early.npz"))
+    ft.add_argument("--out", default=os.path.join(HERE, "data", "kh256_flux_mlp_es.npz"))
+    ft.add_argument("--seeds", default="0,1,2,3")
+    ft.add_argument("--times", default="0.025,0.05,0.1")
+    ft.add_argument("--iters", type=int, default=120)
+    ft.add_argument("--pairs", type=int, default=12)
+    ft.add_argument("--sigma", type=float, default=0.02)
+    ft.add_argument("--lr", type=float, default=0.01)
+    ft.add_argument("--eval-seeds", default="1000,1001,1002")
+    ft.add_argument("--eval-times", default="0.025,0.05,0.1")
+    ft.add_argument("--kappas", default=None)
+    ft.add_argument("--report", default=None)
     a = ap.parse_args(argv)
     from . import hip
     log = lambda m: print(m, flush=True)
     seeds = [int(v) for v in a.eval_seeds.split(",")]
     times = [float(v) for v in a.eval_times.split(",")]
-    kappas = [float(v) for v in a.kappas.split(",")]
-    if a.cmd == "train":
+    kappas = [float(v) for v in a.kappas.split(",")] if a.kappas else None
+    if a.cmd == "finetune":
+        net = hip.Net.load(a.net)
+        kappas = kappas or [net.kappa]
+        ft_seeds = [int(v) for v in a.seeds.split(",")]
+        ft_times = [float(v) for v in a.times.split(",")]
+        t0 = time.time()
+        hist = finetune(net, ft_seeds, ft_times, a.iters, a.pairs, a.sigma, a.lr, log=log)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        with np.load(a.net, allow_pickle=False) as z:
+            meta = {k: z[k] for k in z.files if k not in ("params", "in_scale", "out_scale", "kappa")}
+        net.save(a.out, **meta, ft_seeds=np.array(ft_seeds), ft_times=np.array(ft_times), ft_iters=a.iters,
+                 ft_pairs=a.pairs, ft_sigma=a.sigma, ft_lr=a.lr, ft_history=np.array(hist))
+        log(f"finetune {time.time() - t0:.1f} s, saved {a.out}")
+        rep = {"finetune_s": time.time() - t0, "history": hist, "eval": evaluate(net, seeds, times, kappas, log=log)}
+    elif a.cmd == "train":
         net = hip.Net()
         net.kappa = a.kappa
         t0 = time.time()
