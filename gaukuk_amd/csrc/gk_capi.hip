@@ -510,6 +510,7 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
     if (nz > 1)
         for (int k = 0; k < lz; k++) {
             if (k >= g && k < g + nz) continue;
+            if (hooked(c, k < g ? 4 : 5)) continue;   // filled by the hook (gk_download_cons)
             const int zt = k < g ? c->bc[4] : c->bc[5];
             if (zt < 1 || zt > 4) continue;
             int kind;
@@ -525,88 +526,145 @@ static void active_to_ref(const gk_ctx* c, const double* act, R* ref) {
 }
 
 // ---- boundary hook (gk_set_bd_hook) ------------------------------------------------------
-// Face f's band of `band` active layers: x faces columns [c0, c0 + band) of every row, y faces
-// rows [r0, r0 + band) of every column.
-static void hook_band_origin(const gk_ctx* c, int f, int& c0, int& r0) {
-    const int b = c->hook_band;
-    c0 = f == 1 ? c->nx - b : 0;
-    r0 = f == 3 ? c->ny - b : 0;
+// Indices below are active-relative: cell (i, j, k) of the active box, ghosts at -1, -2, ... and
+// nx, nx + 1, ...  The mirror is the reference layout [5][nz + 2g][ny + 2g][nx + 2g] (2D: one
+// active plane between 2g ghost planes).
+template <class R>
+static R& mirror_at(const gk_ctx* c, int n, int k, int j, int i) {
+    const int g = c->grid.nghost;
+    const size_t lx = c->nx + 2 * g, ly = c->ny + 2 * g, lz = c->nz + 2 * g;
+    return static_cast<R*>(c->hook_mirror)[(((size_t)n * lz + (k + g)) * ly + (j + g)) * lx + (i + g)];
 }
 
-// Write the bands of the hooked faces of `state` (device, [5][ny][nx] of R) into the mirror at
-// their reference positions; with `kick`, after the constant-gravity half-kick of length h.
+// Face f's band: the `band` active layers nearest the face across the whole active extent of the
+// other axes, [lo, hi) per axis (x, y, z).
+static void hook_band_box(const gk_ctx* c, int f, int lo[3], int hi[3]) {
+    const int n[3] = {c->nx, c->ny, c->nz}, a = f / 2, b = c->hook_band;
+    for (int d = 0; d < 3; d++) { lo[d] = 0; hi[d] = n[d]; }
+    if (f & 1) lo[a] = n[a] - b;
+    else hi[a] = b;
+}
+
+// Write the bands of the hooked faces of `state` (device, [5][nz][ny][nx] of R, row pitch `pitch`)
+// into the mirror; with `kick`, after the first half-kick of the step's sources (constant, then
+// point or binary gravity at the cell centre, the binary's second mass at t + dt/2), as the stage
+// kernel applies it when it loads U^n.
 template <class R>
-static gk_status hook_write_bands(gk_ctx* c, const void* state, bool kick, double h) {
-    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, b = c->hook_band;
-    const int lx = nx + 2 * g, ly = ny + 2 * g;
-    R* mir = static_cast<R*>(c->hook_mirror);
+static gk_status hook_write_bands(gk_ctx* c, const void* state, bool kick) {
+    const int ny = c->ny, nz = c->nz, b = c->hook_band, pitch = c->pitch, g = c->grid.nghost;
     const R* dev = static_cast<const R*>(state);
-    const R gx = R(c->phys.g[0]), gy = R(c->phys.g[1]), gz = R(c->phys.g[2]), hh = R(h);
-    for (int f = 0; f < 4; f++) {
+    const double dt = c->hclock->dt;
+    const R gx = R(c->phys.g[0]), gy = R(c->phys.g[1]), gz = R(c->phys.g[2]), h = R(0.5 * dt);
+    // point masses as the stage kernel takes them ({gm, x, y, z, rs}) and the cell centres
+    double m0[5], m1[5];
+    const double* pm[2] = {c->pm[0], c->pm[1]};
+    for (int q = 0; q < 2; q++) {
+        double* d = q ? m1 : m0;
+        d[0] = pm[q][0]; d[1] = pm[q][1]; d[2] = pm[q][2]; d[3] = pm[q][3]; d[4] = pm[q][7];
+    }
+    R bx1 = R(0), by1 = R(0);
+    std::vector<double> xc, yc, zcs;
+    double zc = 0;
+    if (kick && c->npoint) {
+        domain_centres(c, xc, yc, zc);
+        const double dz = (c->grid.zmax - c->grid.zmin) / c->grid.nz;
+        zcs.resize((size_t)nz + 2 * g);
+        for (int k = 0; k < (int)zcs.size(); k++) zcs[k] = nz > 1 ? c->grid.zmin - g * dz + 0.5 * dz + k * dz : zc;
+        if (c->npoint == 2) {
+            const double th = c->omega * (c->hclock->t + 0.5 * dt) + c->theta1;
+            bx1 = R(c->ab * std::cos(th));
+            by1 = R(c->ab * std::sin(th));
+        }
+    }
+    for (int f = 0; f < nfaces(c); f++) {
         if (!hooked(c, f)) continue;
-        int c0, r0;
-        hook_band_origin(c, f, c0, r0);
-        const bool xf = f < 2;
-        const int w = xf ? b : nx, rows = xf ? ny : b;
-        c->hook_stage.resize(sizeof(R) * 5 * (size_t)w * rows);
+        int lo[3], hi[3];
+        hook_band_box(c, f, lo, hi);
+        // one strided copy per variable into a staging box; st(n, k, j, i) is its element
+        size_t width, rows, spitch, off;
+        if (f < 2) {        // b columns of all nz * ny rows (planes of ny rows follow each other)
+            width = b; rows = (size_t)nz * ny; spitch = pitch; off = lo[0];
+        } else if (f < 4) { // b consecutive rows of every plane
+            width = (size_t)b * pitch; rows = nz; spitch = (size_t)c->plane2; off = (size_t)lo[1] * pitch;
+        } else {            // b consecutive planes
+            width = (size_t)b * c->plane2; rows = 1; spitch = width; off = (size_t)lo[2] * c->plane2;
+        }
+        c->hook_stage.resize(sizeof(R) * 5 * width * rows);
         R* st = reinterpret_cast<R*>(c->hook_stage.data());
         for (int n = 0; n < 5; n++)
-            GK_TRY_HIP(c, hipMemcpy2DAsync(st + (size_t)n * w * rows, sizeof(R) * w,
-                                           dev + (size_t)n * c->plane + (size_t)r0 * c->pitch + c0, sizeof(R) * c->pitch,
-                                           sizeof(R) * w, rows, hipMemcpyDeviceToHost, c->stream));
+            GK_TRY_HIP(c, hipMemcpy2DAsync(st + (size_t)n * width * rows, sizeof(R) * width,
+                                           dev + (size_t)n * c->plane + off, sizeof(R) * spitch, sizeof(R) * width,
+                                           rows, hipMemcpyDeviceToHost, c->stream));
         GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
-        for (int j = 0; j < rows; j++)
-            for (int i = 0; i < w; i++) {
-                R u[5];
-                for (int n = 0; n < 5; n++) u[n] = st[((size_t)n * rows + j) * w + i];
-                if (kick) gravity_kick(u, gx, gy, gz, hh);
-                for (int n = 0; n < 5; n++)
-                    mir[(((size_t)n * (1 + 2 * g) + g) * ly + (g + r0 + j)) * lx + (g + c0 + i)] = u[n];
-            }
+        auto at = [&](int n, int k, int j, int i) -> R {
+            size_t o;
+            if (f < 2) o = (((size_t)n * nz + k) * ny + j) * b + (i - lo[0]);
+            else if (f < 4) o = (((size_t)n * nz + k) * b + (j - lo[1])) * pitch + i;
+            else o = (((size_t)n * b + (k - lo[2])) * ny + j) * pitch + i;
+            return st[o];
+        };
+        for (int k = lo[2]; k < hi[2]; k++)
+            for (int j = lo[1]; j < hi[1]; j++)
+                for (int i = lo[0]; i < hi[0]; i++) {
+                    R u[5];
+                    for (int n = 0; n < 5; n++) u[n] = at(n, k, j, i);
+                    if (kick) {
+                        gravity_kick(u, gx, gy, gz, h);
+                        if (c->npoint)
+                            point_kick(u, c->npoint, m0, m1, bx1, by1, h, R(xc[i + g]), R(yc[j + g]), R(zcs[k + g]));
+                    }
+                    for (int n = 0; n < 5; n++) mirror_at<R>(c, n, k, j, i) = u[n];
+                }
     }
     return GK_OK;
 }
 
-// The ghost layer at distance d (0 nearest) from hooked face f, as the mirror holds it, is
-// (n, t) -> value with t the tangential active index.
-template <class R>
-static R hook_ghost(const gk_ctx* c, int f, int n, int d, int t) {
-    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
-    const int lx = nx + 2 * g, ly = ny + 2 * g;
-    const R* mir = static_cast<const R*>(c->hook_mirror);
-    int i, j;
-    if (f < 2) { i = f == 0 ? g - 1 - d : g + nx + d; j = g + t; }
-    else { j = f == 2 ? g - 1 - d : g + ny + d; i = g + t; }
-    return mir[(((size_t)n * (1 + 2 * g) + g) * ly + j) * lx + i];
-}
-
-// Before stage q: mirror the stage input's bands, call the hook, upload the two ghost layers
-// nearest each hooked face into that face's fixed-face array (layouts of gk_set_fixed_boundary:
-// x faces [5][ny][2], y faces [5][2][nx]).
+// Before stage q: mirror the stage input's bands, call the hook, upload the ghost layers the stage
+// reads into each hooked face's fixed-face array (layouts of gk_set_fixed_boundary: x faces
+// [5][nz][ny][2], y faces [5][nz][2][nx], both the two layers nearest the face; z faces
+// [5][ny][nx], the nearest plane).
 template <class R>
 static gk_status hook_refresh_t(gk_ctx* c, size_t q) {
     const StageSpec& s = c->stages[q];
     // forward Euler alternates its two buffers with the clock's `cur`
     const bool swap = c->phys.integrator == 1 && c->hclock->cur;
     const void* in = c->buf[swap ? s.out : s.in];
-    if (gk_status st = hook_write_bands<R>(c, in, s.first && c->gravity, 0.5 * c->hclock->dt)) return st;
+    if (gk_status st = hook_write_bands<R>(c, in, s.first && c->gravity)) return st;
     c->hook(c->hook_user, c->hook_mirror);
-    for (int f = 0; f < 4; f++) {
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    for (int f = 0; f < nfaces(c); f++) {
         if (!hooked(c, f)) continue;
-        const int len = f < 2 ? c->ny : c->nx;
-        c->hook_stage.resize(sizeof(R) * 10 * (size_t)len);
+        const size_t len = f < 2 ? (size_t)10 * nz * ny : f < 4 ? (size_t)10 * nz * nx : (size_t)5 * ny * nx;
+        c->hook_stage.resize(sizeof(R) * len);
         R* st = reinterpret_cast<R*>(c->hook_stage.data());
+        bool finite = true;
+        auto put = [&](size_t o, R v) {
+            finite &= std::isfinite((double)v);
+            st[o] = v;
+        };
         for (int n = 0; n < 5; n++)
-            for (int d = 0; d < 2; d++)
-                for (int t = 0; t < len; t++) {
-                    const R v = hook_ghost<R>(c, f, n, d, t);
-                    if (!std::isfinite((double)v))
-                        return fail(c, GK_EINVAL, "the boundary hook left a non-finite ghost value on face " +
-                                                      std::to_string(f) + " (does the face function read cells "
-                                                      "beyond the mirrored band?)");
-                    st[f < 2 ? ((size_t)n * len + t) * 2 + d : ((size_t)n * 2 + d) * len + t] = v;
+            for (int k = 0; k < nz; k++) {
+                if (f < 4) {
+                    for (int d = 0; d < 2; d++) {
+                        if (f < 2)
+                            for (int j = 0; j < ny; j++)
+                                put((((size_t)n * nz + k) * ny + j) * 2 + d,
+                                    mirror_at<R>(c, n, k, j, f == 0 ? -1 - d : nx + d));
+                        else
+                            for (int i = 0; i < nx; i++)
+                                put((((size_t)n * nz + k) * 2 + d) * nx + i,
+                                    mirror_at<R>(c, n, k, f == 2 ? -1 - d : ny + d, i));
+                    }
+                } else if (k == 0) {
+                    for (int j = 0; j < ny; j++)
+                        for (int i = 0; i < nx; i++)
+                            put(((size_t)n * ny + j) * nx + i, mirror_at<R>(c, n, f == 4 ? -1 : nz, j, i));
                 }
-        GK_TRY_HIP(c, hipMemcpyAsync(c->fix[f], st, sizeof(R) * 10 * (size_t)len, hipMemcpyHostToDevice, c->stream));
+            }
+        if (!finite)
+            return fail(c, GK_EINVAL, "the boundary hook left a non-finite ghost value on face " + std::to_string(f) +
+                                          " (does the face function read cells beyond the mirrored band?)");
+        GK_TRY_HIP(c, hipMemcpyAsync(c->fix[f], st, sizeof(R) * len, hipMemcpyHostToDevice, c->stream));
         GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
     }
     return GK_OK;
@@ -619,26 +677,23 @@ static gk_status hook_refresh(gk_ctx* c, size_t q) {
 // layers over the active tangential range, as the mirror holds them after the call.
 template <class R>
 static gk_status hook_fill_output(gk_ctx* c, const void* state, R* ref) {
-    if (gk_status st = hook_write_bands<R>(c, state, false, 0.0)) return st;
+    if (gk_status st = hook_write_bands<R>(c, state, false)) return st;
     c->hook(c->hook_user, c->hook_mirror);
-    const int g = c->grid.nghost, nx = c->nx, ny = c->ny;
-    const int lx = nx + 2 * g, ly = ny + 2 * g;
+    const int g = c->grid.nghost, nx = c->nx, ny = c->ny, nz = c->nz;
     const R* mir = static_cast<const R*>(c->hook_mirror);
-    const size_t plane_ref = (size_t)(1 + 2 * g) * ly * lx;
-    auto copy = [&](int j, int i) {
-        for (int n = 0; n < 5; n++) {
-            const size_t o = n * plane_ref + ((size_t)g * ly + j) * lx + i;
-            ref[o] = mir[o];
-        }
-    };
-    for (int f = 0; f < 4; f++) {
+    for (int f = 0; f < nfaces(c); f++) {
         if (!hooked(c, f)) continue;
-        for (int d = 0; d < g; d++) {
-            if (f < 2)
-                for (int j = g; j < g + ny; j++) copy(j, f == 0 ? g - 1 - d : g + nx + d);
-            else
-                for (int i = g; i < g + nx; i++) copy(f == 2 ? g - 1 - d : g + ny + d, i);
-        }
+        const int a = f / 2, n[3] = {nx, ny, nz};
+        int lo[3] = {0, 0, 0}, hi[3] = {nx, ny, nz};
+        lo[a] = f & 1 ? n[a] : -g;
+        hi[a] = f & 1 ? n[a] + g : 0;
+        for (int v = 0; v < 5; v++)
+            for (int k = lo[2]; k < hi[2]; k++)
+                for (int j = lo[1]; j < hi[1]; j++)
+                    for (int i = lo[0]; i < hi[0]; i++) {
+                        const size_t o = &mirror_at<R>(c, v, k, j, i) - mir;
+                        ref[o] = mir[o];
+                    }
     }
     return GK_OK;
 }
@@ -737,7 +792,7 @@ gk_status gk_create(const gk_grid* grid, const gk_phys* phys, gk_ctx** out) {
         g_create_error = "Invalid grid parameters from config"; return GK_EINVAL;
     }
     if (grid->nz > 1) {
-        // 3D: boundary hooks, ensembles and the learned correction are 2D only
+        // 3D: ensembles and the learned correction are 2D only
         if (grid->ny < 2) { g_create_error = "3D grids need NY >= 2"; return GK_EINVAL; }
     }
     if (grid->nghost < 2) { g_create_error = "NGhost must be >= 2 for the PLM stencil"; return GK_EINVAL; }
@@ -997,7 +1052,6 @@ gk_status gk_set_point_gravity(gk_ctx* c, double gm, double x, double y, double 
                                 double rs) {
     if (!c) return GK_EINVAL;
     if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
-    if (c->hook) return fail(c, GK_EINVAL, "boundary hooks do not take point gravity");
     // SourceTerm::EnrollPointGravity (src/source_term/source.cpp:33-67): the first mass gives point
     // gravity, a second turns it into binary gravity, a third is an error
     if (c->npoint >= 2) return fail(c, GK_EINVAL, "Setup failed: too many point gravity.");
@@ -1202,26 +1256,45 @@ gk_status gk_set_immersed_shell(gk_ctx* c, double xs, double ys, double zs, doub
 gk_status gk_set_bd_hook(gk_ctx* c, int face_mask, int band, gk_bd_hook hook, void* user, void* mirror) {
     if (!c) return GK_EINVAL;
     if (!hook || !mirror) return fail(c, GK_EINVAL, "gk_set_bd_hook: NULL hook or mirror");
-    if (c->nz > 1) return fail(c, GK_EINVAL, "boundary hooks are 2D only");
-    if (face_mask <= 0 || face_mask > 15) return fail(c, GK_EINVAL, "gk_set_bd_hook: face_mask selects faces 0-3");
+    const int nf = nfaces(c);
+    if (face_mask <= 0 || face_mask >= (1 << nf))
+        return fail(c, GK_EINVAL, nf == 6 ? "gk_set_bd_hook: face_mask selects faces 0-5"
+                                          : "gk_set_bd_hook: face_mask selects faces 0-3 (z faces need NZ > 1)");
     if ((face_mask & 12) && c->ny < 2) return fail(c, GK_EINVAL, "gk_set_bd_hook: y faces of a one-row grid");
-    if (band < 1 || band > c->nx || band > c->ny)
-        return fail(c, GK_EINVAL, "gk_set_bd_hook: band must lie in [1, min(nx, ny)]");
-    if (c->npoint) return fail(c, GK_EINVAL, "boundary hooks do not take point gravity");
+    if (band < 1 || band > c->nx || band > c->ny || (nf == 6 && band > c->nz))
+        return fail(c, GK_EINVAL, "gk_set_bd_hook: band must lie in [1, min(nx, ny, nz)]");
     if (c->net) return fail(c, GK_EINVAL, "a network is attached (gk_set_net): detach it first");
     // a hooked face became a type-4 face; without its hook it would keep its last ghost layers
     if (c->hook && (c->hook_mask & ~face_mask))
         return fail(c, GK_EINVAL, "gk_set_bd_hook: a later call must keep every face an earlier call hooked");
+    // 3D: a z face of type 0 registers the y copy, which UpdateBD runs after both y faces
+    // (boundary.cpp:90-135): it overwrites a hooked y face's ghost layers on its side, so that y
+    // face stays the copy gk_create made it.  A hooked z face replaces the copy, and the y face on
+    // its side gets its own type back.
+    int mask = face_mask;
+    if (nf == 6)
+        for (int s = 0; s < 2; s++) {
+            if ((face_mask >> (4 + s)) & 1) {
+                if (c->bc[4 + s] == 0) {
+                    const int t = c->phys.bc[2 + s];
+                    c->bc[2 + s] = t >= 0 && t <= 4 ? t : 0;
+                }
+            } else if (c->bc[4 + s] == 0) {
+                mask &= ~(4 << s);
+            }
+        }
     GK_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    for (int f = 0; f < 4; f++) {
-        if (!((face_mask >> f) & 1)) continue;
-        if (!c->fix[f]) GK_TRY_HIP(c, hipMalloc(&c->fix[f], c->esz * 10 * (size_t)(f < 2 ? c->ny : c->nx)));
+    for (int f = 0; f < nf; f++) {
+        if (!((mask >> f) & 1)) continue;
+        const size_t len = f < 2 ? (size_t)10 * c->nz * c->ny : f < 4 ? (size_t)10 * c->nz * c->nx
+                                                                     : (size_t)5 * c->ny * c->nx;
+        if (!c->fix[f]) GK_TRY_HIP(c, hipMalloc(&c->fix[f], c->esz * len));
         c->bc[f] = 4;
     }
     c->hook = hook;
     c->hook_user = user;
     c->hook_mirror = mirror;
-    c->hook_mask = face_mask;
+    c->hook_mask = mask;
     c->hook_band = band;
     c->plans.clear();   // the fixed-face kernel variant runs from now on
     return GK_OK;

@@ -309,10 +309,11 @@ class Solver:
         self._check(self.L.gk_set_fixed_boundary(self.h, a.ctypes.data, int(bool(fix_inner))))
 
     def set_bd_hook(self, face_mask: int, band: int, fn):
-        """gk_set_bd_hook: faces in face_mask (bit f: 0 x left .. 3 y right) get their ghost layers
-        from fn(mirror) before every stage, mirror being the reference-layout array
-        [5][NZ+2g][NY+2g][NX+2g] in the solver's precision (NaN except the mirrored bands of
-        `band` active layers next to those faces, which the library writes before each call)."""
+        """gk_set_bd_hook: faces in face_mask (bit f: 0 x left .. 3 y right, and 4 z left, 5 z
+        right on 3D grids) get their ghost layers from fn(mirror) before every stage, mirror being
+        the reference-layout array [5][NZ+2g][NY+2g][NX+2g] in the solver's precision (NaN except
+        the mirrored bands of `band` active layers next to those faces, which the library writes
+        before each call)."""
         dt = np.float64 if self.precision == 64 else np.float32
         mirror = np.full(self._ref_shape(), np.nan, dt)
         hook = BD_HOOK(lambda user, ptr: fn(mirror))

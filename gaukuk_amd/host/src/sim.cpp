@@ -83,11 +83,11 @@ bool Sim::ProbeSelfDefined(int f, TArray<Real>& out) const {
 }
 
 // gk_set_bd_hook's function: the enrolled face functions of the state-dependent self-defined
-// faces, in the reference's UpdateBD order (src/boundary/boundary.cpp:127-135), on the mirror
-// array, whose active cells outside the mirrored bands hold NaN.
+// faces, in the reference's UpdateBD order (src/boundary/boundary.cpp:127-142: x, y, then z), on
+// the mirror array, whose active cells outside the mirrored bands hold NaN.
 void Sim::BdHook(void* user, void*) {
     Sim* s = static_cast<Sim*>(user);
-    for (int f = 0; f < 4; f++)
+    for (int f = 0; f < 6; f++)
         if (s->hookFaces_ & (1 << f)) s->boundary.Face(f)(s->hookMirror_, s->grid, s->eos);
 }
 
@@ -107,18 +107,13 @@ void Sim::EnsureDevice() {
         p.stepmax = stepMax;
         for (int f = 0; f < 6; f++) p.bc[f] = boundary.type[f];
         // state-independent self-defined faces become fixed faces holding the ghost values their
-        // function writes; state-dependent ones (2D) are refreshed before every stage through the
+        // function writes; state-dependent ones are refreshed before every stage through the
         // boundary hook
-        static const char* names[6] = {"x left", "x right", "y left", "y right", "z left", "z right"};
         const int nface = grid.nz > 1 ? 6 : 4;
         for (int f = 0; f < nface; f++) {
             if (!boundary.selfDefined[f]) continue;
             TArray<Real> ghosts;
             if (!ProbeSelfDefined(f, ghosts)) {
-                if (grid.nz > 1)
-                    throw std::runtime_error(std::string("the self-defined boundary on the ") + names[f] +
-                                             " face depends on the state: in 3D only state-independent "
-                                             "self-defined faces run on the device");
                 hookFaces_ |= 1 << f;
                 p.bc[f] = 4;
                 continue;

@@ -39,8 +39,8 @@ typedef enum {
 
 /* Grid: the keys NX, NY, NZ, NGhost, xmin..zmax of the .in file (src/grid.hpp:18-41,
  * src/sim.cpp:16-42).  nz = 1 runs the 1D/2D path, nz > 1 the 3D path (ny >= 2; every face type,
- * constant and point gravity, the inner fixed region and the immersed sphere; boundary hooks are
- * 2D only); nghost must be at least 2 (PLM stencil). */
+ * constant and point gravity, the inner fixed region, the immersed sphere and boundary hooks);
+ * nghost must be at least 2 (PLM stencil). */
 typedef struct {
     int nx, ny, nz, nghost;
     double xmin, xmax, ymin, ymax, zmin, zmax;
@@ -89,20 +89,25 @@ gk_status gk_set_immersed_shell(gk_ctx* ctx, double x, double y, double z, doubl
 
 /* Self-defined faces whose ghost values depend on the state (Boundary::EnrollSelfDefineBD*,
  * src/boundary/boundary.hpp:28-45; the reference calls them in every UpdateBD, i.e. before every
- * stage: src/evolution/rk2.cpp:12,17, boundary.cpp:127-135).  The faces in `face_mask` (bit f for
- * face f: 0 x left, 1 x right, 2 y left, 3 y right; 2D grids) become fixed (type-4) faces whose
- * two ghost layers nearest the face are refreshed before every stage: the library writes the
- * `band` active layers nearest each of those faces of the stage's input (after the first gravity
- * half-kick in the first stage, as the reference's face functions see it) into `mirror` at their
- * reference-layout positions, calls hook(user, mirror), and reads the ghost layers back from
- * `mirror`.  `mirror` is the caller's host array Real[5][lenz][leny][lenx] in the context's
- * precision; the library writes only those bands, so cells the caller poisoned (e.g. with NaN)
- * stay poisoned, and a face function that reads beyond the band yields non-finite ghosts, which
- * fail the step with GK_EINVAL.  Stepping returns to the host once per stage; gk_download_cons
- * fills the hooked faces' ghost cells by calling the hook on the current state.  Fails with
- * GK_EINVAL for 3D grids, a NULL hook or mirror, band outside [1, min(nx, ny)], y faces of a
- * one-row grid, point gravity or an attached network.  A later call replaces the hook; its mask
- * must include every face an earlier call hooked (GK_EINVAL otherwise). */
+ * stage: src/evolution/rk2.cpp:12,17, boundary.cpp:127-142).  The faces in `face_mask` (bit f for
+ * face f: 0 x left, 1 x right, 2 y left, 3 y right, and on 3D grids 4 z left, 5 z right) become
+ * fixed (type-4) faces whose ghost layers the stage reads (the two nearest an x or y face, the
+ * nearest plane of a z face, over the active tangential range) are refreshed before every stage:
+ * the library writes the `band` active layers nearest each of those faces of the stage's input
+ * (after the first half-kick of constant, point or binary gravity in the first stage, as the
+ * reference's face functions see it) into `mirror` at their reference-layout positions, calls
+ * hook(user, mirror), and reads the ghost layers back from `mirror`.  `mirror` is the caller's host
+ * array Real[5][lenz][leny][lenx] in the context's precision; the library writes only those bands,
+ * so cells the caller poisoned (e.g. with NaN) stay poisoned, and a face function that reads beyond
+ * the band yields non-finite ghosts, which fail the step with GK_EINVAL.  Ghost cells outside the
+ * active tangential range (edges and corners) are not taken from the mirror: they keep what the
+ * reference's own faces leave there.  In 3D a y face on the side of a type-0 z face stays the copy
+ * that z face registers (the reference runs it after the y faces), hooked or not.  Stepping
+ * returns to the host once per stage; gk_download_cons fills the hooked faces' ghost cells by
+ * calling the hook on the current state.  Fails with GK_EINVAL for a NULL hook or mirror, a mask
+ * naming z faces of a 2D grid, band outside [1, min(nx, ny, nz)] (2D: [1, min(nx, ny)]), y faces
+ * of a one-row grid or an attached network.  A later call replaces the hook; its mask must include
+ * every face an earlier call hooked (GK_EINVAL otherwise). */
 typedef void (*gk_bd_hook)(void* user, void* mirror);
 gk_status gk_set_bd_hook(gk_ctx* ctx, int face_mask, int band, gk_bd_hook hook, void* user, void* mirror);
 

@@ -268,6 +268,23 @@ PROJECT_SETUPS = {
                     dataType=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0, NX=80, NY=72, NZ=1,
                     NGhost=2, xleft=0, xright=0, yleft=0, yright=0, zleft=0, zright=0, rho0=1.0, p0=1.0,
                     p_ratio=20.0, r0=0.15, x0=0.3, y0=0.35, vshear=0.4, lidU=0.3),
+    # tests/golden/setup_statebd3d.cpp: state-dependent self-defined faces on all six sides (3D),
+    # then under a point mass in 3D and in 2D (NZ = 1: the four x and y faces)
+    "statebd3d": dict(gamma=1.4, tmax=1e30, stepmax=30, CFL=0.4, integrator=3, RcOrder=2, dtoutput=1e30,
+                      dataType=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0, NX=40, NY=36,
+                      NZ=30, NGhost=2, xleft=0, xright=0, yleft=0, yright=0, zleft=0, zright=0, rho0=1.0,
+                      p0=1.0, p_ratio=20.0, r0=0.2, x0=0.35, y0=0.4, z0=0.6, vshear=0.4, vz0=0.3, lidU=0.3,
+                      gm=0.0),
+    "statebd3d_pg": dict(setup="statebd3d", gamma=1.4, tmax=1e30, stepmax=30, CFL=0.4, integrator=2, RcOrder=2,
+                         dtoutput=1e30, dataType=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                         NX=36, NY=40, NZ=28, NGhost=2, xleft=0, xright=0, yleft=0, yright=0, zleft=0, zright=0,
+                         rho0=1.0, p0=1.0, p_ratio=10.0, r0=0.2, x0=0.4, y0=0.35, z0=0.45, vshear=0.3, vz0=0.2,
+                         lidU=0.2, gm=0.05, xg=0.5, yg=0.5, zg=0.5, rs=0.1),
+    "statebd_pg": dict(setup="statebd3d", gamma=1.4, tmax=1e30, stepmax=60, CFL=0.4, integrator=3, RcOrder=2,
+                       dtoutput=1e30, dataType=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                       NX=80, NY=72, NZ=1, NGhost=2, xleft=0, xright=0, yleft=0, yright=0, zleft=0, zright=0,
+                       rho0=1.0, p0=1.0, p_ratio=20.0, r0=0.15, x0=0.3, y0=0.35, z0=0.5, vshear=0.4, vz0=0.0,
+                       lidU=0.3, gm=0.05, xg=0.55, yg=0.5, zg=0.5, rs=0.08),
 }
 
 
@@ -275,9 +292,10 @@ def project_setups():
     """This project's plug-in setups, built into the reference, on their .in values."""
     for name, vals in PROJECT_SETUPS.items():
         vals = dict(vals)
+        setup = vals.pop("setup", name)
         with tempfile.TemporaryDirectory() as d:
             write_in(os.path.join(d, "case.in"), vals)
-            r = subprocess.run([os.path.join(REF, f"gaukuk_ref_{name}"), "-i", "case.in"], cwd=d,
+            r = subprocess.run([os.path.join(REF, f"gaukuk_ref_{setup}"), "-i", "case.in"], cwd=d,
                                env=dict(os.environ, OMP_NUM_THREADS="4"), capture_output=True, text=True,
                                check=True)
             steps = int(re.search(r"total steps =\s*(\d+)", r.stdout).group(1))
@@ -285,7 +303,7 @@ def project_setups():
             outs = sorted(f for f in os.listdir(d) if f.startswith(kind + "_"))
             last = GaukukOutput(os.path.join(d, outs[-1]))
             np.savez_compressed(os.path.join(HERE, f"setup_{name}.npz"), final=last.active().copy(),
-                                t=np.array(float(last.t)), steps=np.array(steps), kind=kind, setup=name,
+                                t=np.array(float(last.t)), steps=np.array(steps), kind=kind, setup=setup,
                                 meta=json.dumps(vals))
         print(f"project setup {name:10s} {vals['NX']}x{vals['NY']} steps={steps} t={float(last.t)!r} ({kind})")
 

@@ -28,7 +28,10 @@ SETUPS = ["blast", "disk", "disk_planet", "kh", "rt", "shock_tube", "solid_body"
           "shock_tube3d",
           # this project's tests/golden/setup_statebd.cpp in the same plug-in format: self-defined
           # faces whose ghost values depend on the state on all four sides (the boundary hook)
-          "statebd"]
+          "statebd",
+          # tests/golden/setup_statebd3d.cpp: the same on all six sides of a 3D grid, then under a
+          # point mass in 3D and in 2D
+          "statebd3d", "statebd3d_pg", "statebd_pg"]
 
 
 def rel_l1(a, b):
