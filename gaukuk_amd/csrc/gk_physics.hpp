@@ -7,8 +7,8 @@
 // __host__ __device__ so host code and kernels share one definition.
 //
 // Arithmetic follows the reference's operation order so that the FP64 path differs from the
-// reference only by FMA contraction and reassociation-free rounding (parity target 1e-12
-// relative L1).  Comparisons use std::min/std::max semantics (second operand returned only when
+// reference only by FMA contraction and by the few-ulp quotient and root cores below (parity
+// target 1e-12 relative L1).  Comparisons use std::min/std::max semantics (second operand returned only when
 // strictly better), not fmin/fmax, to keep the reference's signed-zero and NaN behaviour, except
 // for maxnum() below, used only where the two agree for every result that matters.
 #pragma once
@@ -50,78 +50,48 @@ template <class R> __host__ __device__ __forceinline__ R maxnum(R a, R b) {
 }
 template <> __host__ __device__ __forceinline__ float maxnum<float>(float a, float b) { return smax(a, b); }
 
-template <class R> __host__ __device__ __forceinline__ R gsqrt(R x);
-template <> __host__ __device__ __forceinline__ double gsqrt<double>(double x) { return __builtin_sqrt(x); }
-template <> __host__ __device__ __forceinline__ float gsqrt<float>(float x) { return __builtin_sqrtf(x); }
-
-// FP64 quotient and square root for operands in the normal range.  The IEEE sequences the
-// compiler emits wrap a core of (reciprocal, two Newton steps, residual correction) for the
-// quotient and (reciprocal square root, Goldschmidt step, two Newton corrections) for the root in
-// range-scaling and special-value steps (div_scale/div_fixup, ldexp and class tests).  For
-// normal finite operands whose quotient is normal those steps are the identity.  The cores below
-// also take one refinement step less than the compiler's: the quotient one Newton step on the
-// hardware reciprocal before the residual correction, the root one Newton correction after the
-// Goldschmidt step.  The step they drop only shrinks an error that the remaining correction
-// already rounds away (its relative error is about the square of the one left after the last
-// refinement): on the device the results equal the IEEE operators' bit for bit on 3 x 64M draws
-// (binary exponents within +-4, +-30 and +-500), 5 (quotient) and 10 (root) instructions fewer
-// than the compiler's sequences.  gk_selftest() checks the identity on the device.  The host build
-// uses the plain operators.
-__host__ __device__ __forceinline__ double qdiv(double n, double d) {
+// Quotients and roots of the stencil arithmetic, for operands in the normal range (floored
+// densities and pressures, sums of them).  The IEEE sequences the compiler emits wrap a core of
+// hardware reciprocal (reciprocal square root) plus refinement steps in range-scaling and
+// special-value steps (div_scale/div_fixup, ldexp and class tests), which are the identity for
+// such operands.  The cores below keep one refinement: FP64 v_rcp_f64 / v_rsq_f64 are good to
+// about 2^-25, and one residual correction (quotient), one Newton step (reciprocal) or one
+// Goldschmidt step (roots) takes that to a few ulp.  Largest errors against the IEEE operations
+// measured on the device over 3 x 64M draws (exponents within +-4, +-30 and +-500): FP64 quotient
+// 19 ulp, reciprocal 11, root 37, sqrt(a / b) 38, 1 / sqrt 20; FP32 (the hardware v_rcp_f32,
+// v_sqrt_f32, v_rsq_f32) 1-2 ulp.  gk_selftest() counts device results beyond twice those bounds.
+// The FP64 path is held to 1e-12 relative L1 against the reference per conserved variable (KH
+// 2048^2 after 200 RK3 steps: 1.2e-13 with these cores, 3.3e-14 with correctly rounded ones, for
+// 6% more VALU instructions per stage); FP32 is judged by its L1 distance from the FP64 run.  The
+// host build uses the plain operators.
+__host__ __device__ __forceinline__ double srcp(double d) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    double r = __builtin_amdgcn_rcp(d);
-    const double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(d);
+    return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+#else
+    return 1.0 / d;
+#endif
+}
+__host__ __device__ __forceinline__ double sdiv(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rcp(d);
     const double q = n * r;
-    const double rem = __builtin_fma(-d, q, n);
-    return __builtin_fma(rem, r, q);
+    return __builtin_fma(__builtin_fma(-d, q, n), r, q);
 #else
     return n / d;
 #endif
 }
-// FP32: the same core as the compiler's IEEE sequence (reciprocal, one Newton step, quotient, two
-// residual corrections) without its range scaling and special-value fix-up, which are the identity
-// for normal operands with a normal quotient: same bits, 3 fewer instructions.
-__host__ __device__ __forceinline__ float qdiv(float n, float d) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    float r = __builtin_amdgcn_rcpf(d);
-    const float e = __builtin_fmaf(-d, r, 1.0f);
-    r = __builtin_fmaf(e, r, r);
-    float q = n * r;
-    float rem = __builtin_fmaf(-d, q, n);
-    q = __builtin_fmaf(rem, r, q);
-    rem = __builtin_fmaf(-d, q, n);
-    return __builtin_fmaf(rem, r, q);
-#else
-    return n / d;
-#endif
-}
-
-__host__ __device__ __forceinline__ double qsqrt(double x) {
+// Goldschmidt on y = rsq(x): g = x y ~ sqrt(x), h = y / 2 ~ 1 / (2 sqrt(x)), r = 1/2 - g h;
+// sqrt(x) ~ g + g r and 1 / sqrt(x) ~ 2 (h + h r) = y + y r.
+__host__ __device__ __forceinline__ double ssqrt(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y;
-    double h = y * 0.5;
-    const double r = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, r, g);
-    h = __builtin_fma(h, r, h);
-    const double d = __builtin_fma(-g, g, x);
-    return __builtin_fma(d, h, g);
+    const double g = x * y, h = 0.5 * y;
+    return __builtin_fma(g, __builtin_fma(-h, g, 0.5), g);
 #else
     return __builtin_sqrt(x);
 #endif
 }
-__host__ __device__ __forceinline__ float qsqrt(float x) { return __builtin_sqrtf(x); }
-
-// Quotients and roots of the stencil arithmetic.  FP64 keeps the exact IEEE results above (the
-// FP64 path is compared with the reference at 1e-12).  FP32 is judged by its L1 distance from the
-// FP64 run, not bit for bit, so it uses the hardware reciprocal, square root and reciprocal square
-// root (v_rcp_f32, v_sqrt_f32, v_rsq_f32: about 1 ulp) in place of the IEEE sequences (8-12
-// instructions each); operands here are normal numbers (floored densities and pressures, sums of
-// them).  The host build uses the plain operators.
-__host__ __device__ __forceinline__ double srcp(double d) { return qdiv(1.0, d); }
-__host__ __device__ __forceinline__ double sdiv(double n, double d) { return qdiv(n, d); }
-__host__ __device__ __forceinline__ double ssqrt(double x) { return qsqrt(x); }
 __host__ __device__ __forceinline__ float srcp(float d) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_rcpf(d);
@@ -137,9 +107,17 @@ __host__ __device__ __forceinline__ float ssqrt(float x) {
     return __builtin_sqrtf(x);
 #endif
 }
-// sqrt(a / b) for positive normal a, b: FP64 the IEEE root of the IEEE quotient (the reference's
-// SoundSpeed order); FP32 a * rsq(a * b), one hardware reciprocal square root.
-__host__ __device__ __forceinline__ double sqrt_ratio(double a, double b) { return qsqrt(qdiv(a, b)); }
+// sqrt(a / b) for positive normal a, b (the sound speed): a / sqrt(a b), one reciprocal square
+// root instead of a quotient and a root.
+__host__ __device__ __forceinline__ double sqrt_ratio(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double z = a * b, y = __builtin_amdgcn_rsq(z);
+    const double g = z * y, h = 0.5 * y;
+    return a * __builtin_fma(y, __builtin_fma(-h, g, 0.5), y);
+#else
+    return __builtin_sqrt(a / b);
+#endif
+}
 __host__ __device__ __forceinline__ float sqrt_ratio(float a, float b) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return a * __builtin_amdgcn_rsqf(a * b);
@@ -147,8 +125,15 @@ __host__ __device__ __forceinline__ float sqrt_ratio(float a, float b) {
     return __builtin_sqrtf(a / b);
 #endif
 }
-// 1 / sqrt(x): FP64 the IEEE quotient of the IEEE root; FP32 the hardware reciprocal square root.
-__host__ __device__ __forceinline__ double srsqrt(double x) { return qdiv(1.0, qsqrt(x)); }
+__host__ __device__ __forceinline__ double srsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    const double g = x * y, h = 0.5 * y;
+    return __builtin_fma(y, __builtin_fma(-h, g, 0.5), y);
+#else
+    return 1.0 / __builtin_sqrt(x);
+#endif
+}
 __host__ __device__ __forceinline__ float srsqrt(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_rsqf(x);

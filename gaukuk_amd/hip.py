@@ -652,7 +652,8 @@ class Ensemble:
 
 
 def selftest(count: int = 1 << 24, seed: int = 12345, emax: int = 500) -> int:
-    """Bitwise check of the kernels' short FP64 quotient/root against IEEE; returns mismatches."""
+    """gk_selftest: the kernels' quotient/root cores against IEEE within their ulp bounds, and the
+    DPP lane shifts against cross-lane shuffles; returns the number of failures."""
     bad = C.c_longlong()
     st = lib().gk_selftest(count, seed, emax, C.byref(bad))
     if st != 0:

@@ -178,12 +178,13 @@ gk_status gk_check_finite(gk_ctx* ctx);
 gk_status gk_set_tile(gk_ctx* ctx, int seg_rows);
 gk_status gk_get_tile(gk_ctx* ctx, int* seg_rows, int* strips, int* waves);
 
-/* Device self-check, no counterpart in the reference: the short FP64 quotient, reciprocal and
- * square-root sequences of the stage kernels against the IEEE operators on `count` operand draws
- * with binary exponents in [-emax, emax] (root operands [-2*emax, 2*emax]), and the FP32 quotient
- * core; *mismatches counts results whose bits differ; also the stage kernels' neighbour-lane moves
- * against the cross-lane shuffles on every lane that has a source lane (one draw per launched
- * thread).  Expected 0 for emax <= 500. */
+/* Device self-check, no counterpart in the reference: the stage kernels' quotient, reciprocal,
+ * square-root, sqrt(a / b) and reciprocal-square-root cores (FP64 and FP32) against the IEEE
+ * operators on `count` operand draws with binary exponents in [-emax, emax] (root operands
+ * [-2*emax, 2*emax]; FP32 within +-60); *mismatches counts results farther from the IEEE result
+ * than the core's bound (FP64 16-64 ulp, FP32 4 ulp: about twice the largest error measured), and
+ * the stage kernels' neighbour-lane moves that differ from the cross-lane shuffles on a lane that
+ * has a source lane (one draw per launched thread).  Expected 0 for emax <= 500. */
 gk_status gk_selftest(long long count, unsigned long long seed, int emax, long long* mismatches);
 
 /* ---- Ensembles: many independent grids advanced together (no counterpart in the reference,

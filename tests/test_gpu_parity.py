@@ -150,10 +150,11 @@ def test_download_ref_layout_ghosts(hip):
 
 
 @pytest.mark.parametrize("emax", [8, 100, 500])
-def test_short_div_sqrt_bitwise(hip, emax):
-    """The stage kernels' FP64 quotient/root and FP32 quotient sequences return the IEEE bits for
-    normal operands (FP32: every divisor mantissa, eight times), and their DPP lane shifts move
-    the same values as the cross-lane shuffles."""
+def test_quotient_root_cores_within_bounds(hip, emax):
+    """The stage kernels' quotient, reciprocal and root cores stay within their ulp bounds of the
+    IEEE results for normal operands (FP64 16-64 ulp, FP32 4 ulp; FP32 divisors run through every
+    mantissa, eight times), and their DPP lane shifts move the same values as the cross-lane
+    shuffles.  A core that lost its refinement step is off by about 2^27 ulp in FP64."""
     assert hip.selftest(count=1 << 26, seed=7 + emax, emax=emax) == 0
 
 
