@@ -33,9 +33,9 @@ NX = NY = 2048
 # The three RK3 stage kernels of a step, with the algorithmic HBM bytes per cell of one launch
 # (FP64): the first stage reads U^n and writes U1 (80 B); the two combine stages read U_s and
 # U^n and write one state (120 B).
-STAGES = [("stage_kernel<double, VanLeer, THREE=false, GRAV=false, LAST=false, VAR=0> (RK3 stage 1)", 80.0),
-          ("stage_kernel<double, VanLeer, THREE=true, GRAV=false, LAST=false, VAR=0> (RK3 stage 2)", 120.0),
-          ("stage_kernel<double, VanLeer, THREE=true, GRAV=false, LAST=true, VAR=0> (RK3 stage 3)", 120.0)]
+STAGES = [("stage_kernel<double, VanLeer, THREE=false, GRAV=false, LAST=false, VAR=4> (RK3 stage 1)", 80.0),
+          ("stage_kernel<double, VanLeer, THREE=true, GRAV=false, LAST=false, VAR=4> (RK3 stage 2)", 120.0),
+          ("stage_kernel<double, VanLeer, THREE=true, GRAV=false, LAST=true, VAR=4> (RK3 stage 3)", 120.0)]
 
 
 def log(msg):

@@ -346,14 +346,23 @@ static void launch_stage_f(gk_ctx* c, const StageArgs<R>& a, bool last) {
     hipLaunchKernelGGL(k, dim3(p.nblocks), dim3(kBlock), 0, c->stream, b);
 }
 
+// One state larger than half of the 256 MB memory-side cache: two of a step's states no longer fit
+// together, and plain 2D PLM grids run the stage variant with cache-policy hints (VAR 4).
+static bool states_exceed_cache(const gk_ctx* c) { return (size_t)c->plane * 5 * c->esz > (size_t(128) << 20); }
+
 template <class R, class L, bool THREE>
 static void launch_stage_t(gk_ctx* c, const StageArgs<R>& a, bool last) {
     const bool saved = has_fixed_face(c) || c->inner;
     if (c->nz > 1) {
         if (saved) launch_stage_f<R, L, THREE, 3>(c, a, last);
         else launch_stage_f<R, L, THREE, 2>(c, a, last);
-    } else if (saved) launch_stage_f<R, L, THREE, 1>(c, a, last);
-    else launch_stage_f<R, L, THREE, 0>(c, a, last);
+    } else if (saved) {
+        launch_stage_f<R, L, THREE, 1>(c, a, last);
+    } else {
+        if constexpr (L::order == 2)
+            if (states_exceed_cache(c)) return launch_stage_f<R, L, THREE, 4>(c, a, last);
+        launch_stage_f<R, L, THREE, 0>(c, a, last);
+    }
 }
 
 template <class R>

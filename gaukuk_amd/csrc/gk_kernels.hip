@@ -75,15 +75,18 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 // step's max wave speed).  THREE: the three-coefficient combine of RK stages 2 and 3.  VAR selects
 // the boundary/geometry variant, so that plain 2D grids run code with no trace of the others:
 // 0 plain 2D; 1 2D with fixed faces or the inner fixed region (saved data); 2 3D (nz > 1); 3 3D
-// with saved data.
+// with saved data; 4 plain 2D whose states are too large for two of them to stay in the
+// memory-side cache (cache-policy hints on the state streams, below).
 // One wave advances the cells of strip `strip`, segment `seg` of one grid by one stage.
 // `strip` and `seg` are wave-uniform (scalar registers), so row logic is scalar branches and
 // scalar address arithmetic.
+constexpr bool is3d(int var) { return var == 2 || var == 3; }
+
 template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR, bool PACE = true>
 __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg, int lane, int kp = 0,
                                       R* zring = nullptr, R* oring = nullptr) {
     constexpr bool FIXED = VAR == 1 || VAR == 3;
-    constexpr bool THREED = VAR >= 2;
+    constexpr bool THREED = is3d(VAR);
     constexpr bool KICK_IN = GRAV && !THREE;  // the stage input is U^n
     constexpr bool KICK_BASE = GRAV && THREE; // the combine base is U^n
     DevClock* clk = a.clock;
@@ -152,6 +155,21 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
     auto at = [&](const R* row, int n, unsigned off) -> const R& {
         return *reinterpret_cast<const R*>(reinterpret_cast<const char*>(row) + n * Pb + off);
     };
+    // Cache policy of the state streams (VAR 4).  A step's states pass through the 256 MB
+    // memory-side cache; when one state is larger than half of it (KH 2048^2 FP64: 168 MB) the
+    // combine stages load their input (U1, U2), read for the last time, non-temporally, and the
+    // first stage stores its output (U1 of a multi-stage integrator, read once by the next stage)
+    // non-temporally, which leaves the cache to U^n, read again by every stage of the step.
+    // Measured at KH 2048^2 FP64 RK3 against default loads and stores: stage 2 123 -> 102 us, step
+    // +7.5%; hints on U^n's last read or on the other stores were no better.  Grids whose states
+    // mostly fit lose by them (RT 1024x3072, 126 MB FP64 and 63 MB FP32: 2%), and so does 3D,
+    // where waves also read their neighbour planes' rows: those run with the default policy.
+    constexpr bool NT_IN = VAR == 4 && THREE;
+    constexpr bool NT_OUT = VAR == 4 && !THREE && !LAST;
+    auto in_at = [&](const R* row, int n) -> R {
+        if (NT_IN) return __builtin_nontemporal_load(&at(row, n, off_in));
+        return at(row, n, off_in);
+    };
     // fixed (type 4) x faces: only the edge strips hold their ghost lanes
     const bool xfixed = FIXED && xedge && (a.bc[0] == 4 || a.bc[1] == 4);
     const int nzf = THREED ? a.nz : 1;   // planes of the saved x and y face layers
@@ -187,7 +205,7 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         } else {
             const R* row = in + (long long)rs * pitch;
 #pragma unroll
-            for (int n = 0; n < 5; n++) u[n] = at(row, n, off_in);
+            for (int n = 0; n < 5; n++) u[n] = in_at(row, n);
         }
         if (xfixed && kx == 4) {
             // a ghost column of a fixed x face (cs is its distance from the face); corner cells
@@ -387,7 +405,11 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
         if (owner) {
             char* row = reinterpret_cast<char*>(out + o) + off_own;
 #pragma unroll
-            for (int n = 0; n < 5; n++) *reinterpret_cast<R*>(row + n * Pb) = res[n];
+            for (int n = 0; n < 5; n++) {
+                R* dst = reinterpret_cast<R*>(row + n * Pb);
+                if (NT_OUT) __builtin_nontemporal_store(res[n], dst);
+                else *dst = res[n];
+            }
         }
     };
     // 2D: the raw stage-input rows this wave loads, at its own column, stay in a 4-row ring in the
@@ -532,14 +554,14 @@ __device__ __forceinline__ void march(const StageArgs<R>& a, int strip, int seg,
 // w / strips.
 // 3D waves hold the z-face work on top of the 2D march: two waves per SIMD keep it out of scratch.
 template <class R, class Lim, bool THREE, bool GRAV, bool LAST, int VAR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VAR >= 2 ? 2 : GK_WAVES_PER_EU)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(is3d(VAR) ? 2 : GK_WAVES_PER_EU)))
 void stage_kernel(const StageArgs<R> a) {
     if (!a.clock->active) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int w = xcd_logical_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wid;
     if (w >= a.nwaves) return;
-    if constexpr (VAR >= 2) {
+    if constexpr (is3d(VAR)) {
         // 3D: strip fastest, then segment, then plane (waves of neighbouring planes share an XCD);
         // each wave keeps its neighbour-plane row ring in its own LDS slice
         __shared__ R zring[kBlock / kWave][2 * 3 * 5 * kWave];
