@@ -668,6 +668,7 @@ struct EnsArgs {
     int* counter;         // this stage's work-queue heads, one per XCD (reset by ens_step_begin_kernel)
     int nprob, total_items;
     int xcd_begin[kEnsQueues + 1];   // queue x hands out items [xcd_begin[x], xcd_begin[x + 1])
+    int reverse;          // hand the queue's items out from its last one down
     int in_buf, base_buf, out_buf;
     StageArgs<R> proto;   // stage constants (coefficients, flags, gravity, EOS)
 };
@@ -679,15 +680,17 @@ void ens_stage_kernel(const EnsArgs<R> e) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     __shared__ R oring[kBlock / kWave][4 * 5 * kWave];
     const int q = blockIdx.x % kEnsQueues;
-    const int lo = e.xcd_begin[q], hi = e.xcd_begin[q + 1];
+    const int first = e.xcd_begin[q], count = e.xcd_begin[q + 1] - first;
     int* head = e.counter + q;
-    int item = 0;
-    if (lane == 0) item = lo + atomicAdd(head, 1);
-    item = __builtin_amdgcn_readfirstlane(item);
-    while (item < hi) {
+    // the k-th claim of a queue takes its item first + k, or first + count - 1 - k in reverse
+    int k = 0;
+    if (lane == 0) k = atomicAdd(head, 1);
+    k = __builtin_amdgcn_readfirstlane(k);
+    while (k < count) {
+        const int item = e.reverse ? first + count - 1 - k : first + k;
         // claim the next item before working on this one, to hide the atomic's latency
         int next = 0;
-        if (lane == 0) next = lo + atomicAdd(head, 1);
+        if (lane == 0) next = atomicAdd(head, 1);
         next = __builtin_amdgcn_readfirstlane(next);
         // the problem owning `item`: last p with item_base <= item (binary search, scalar loads)
         int lo = 0, hi = e.nprob - 1;
@@ -712,7 +715,7 @@ void ens_stage_kernel(const EnsArgs<R> e) {
             march<R, Lim, THREE, GRAV, LAST, 0, false>(a, local % d.strips, local / d.strips, lane, 0, nullptr,
                                                        oring[wid]);
         }
-        item = next;
+        k = next;
     }
 }
 
