@@ -20,6 +20,9 @@ Usage (writes the trained network as .npz):
     python -m gaukuk_amd.learned evaluate --net gaukuk_amd/data/kh256_flux_mlp.npz
     python -m gaukuk_amd.learned finetune --net gaukuk_amd/data/kh256_flux_mlp_early.npz \
         --out gaukuk_amd/data/kh256_flux_mlp_es.npz
+A fine-tuned network given as --net continues its fine-tuning (the saved record adds up the
+iterations and joins the objective histories); the shipped kh256_flux_mlp_es.npz is 120 + 260 +
+300 + 400 + 600 iterations from the early network.
 """
 from __future__ import annotations
 
@@ -330,8 +333,14 @@ def main(argv=None) -> int:
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         with np.load(a.net, allow_pickle=False) as z:
             meta = {k: z[k] for k in z.files if k not in ("params", "in_scale", "out_scale", "kappa")}
-        net.save(a.out, **meta, ft_seeds=np.array(ft_seeds), ft_times=np.array(ft_times), ft_iters=a.iters,
-                 ft_pairs=a.pairs, ft_sigma=a.sigma, ft_lr=a.lr, ft_history=np.array(hist))
+        # a network fine-tuned before continues that record: iterations add up and the histories
+        # are joined (the earlier run's settings are replaced by this run's)
+        prev_iters = int(meta.get("ft_iters", 0))
+        prev_hist = np.asarray(meta.get("ft_history", np.zeros((0, 2))), dtype=np.float64).reshape(-1, 2)
+        meta = {k: v for k, v in meta.items() if not k.startswith("ft_")}
+        hist_all = np.concatenate([prev_hist, np.array([[prev_iters + h[0], h[1]] for h in hist], dtype=np.float64)])
+        net.save(a.out, **meta, ft_seeds=np.array(ft_seeds), ft_times=np.array(ft_times),
+                 ft_iters=prev_iters + a.iters, ft_pairs=a.pairs, ft_sigma=a.sigma, ft_lr=a.lr, ft_history=hist_all)
         log(f"finetune {time.time() - t0:.1f} s, saved {a.out}")
         rep = {"finetune_s": time.time() - t0, "history": hist, "eval": evaluate(net, seeds, times, kappas, log=log)}
     elif a.cmd == "train":

@@ -2,9 +2,11 @@
 (training used seeds 0..63).
 
 gaukuk_amd/data/kh256_flux_mlp_es.npz (the early network fine-tuned a posteriori by evolution
-strategies on whole-run errors at t = 0.025, 0.05, 0.1, seeds 0..3; gaukuk_amd.learned.finetune):
-* Over a run on the held-out seeds 1000-1002 the gain now lasts: 10-14 % (mass) and 30-34 %
-  (m_x, m_y, E) at t = 0.025, 6-8 % and 24-34 % at t = 0.05, 1-3 % and 7-14 % at t = 0.1.
+strategies on whole-run errors at t = 0.025, 0.05, 0.1, seeds 0..3, 1680 iterations;
+gaukuk_amd.learned.finetune):
+* Over a run on the held-out seeds 1000-1002 the gain lasts: 29-31 % (mass), 33-35 % (m_x),
+  35-36 % (m_y), 44-45 % (E) at t = 0.025; 26-30, 29-30, 39-41, 42-44 % at t = 0.05; 8-9, 10-16,
+  15-18, 21-22 % at t = 0.1.
 
 gaukuk_amd/data/kh256_flux_mlp_early.npz (trained on frames of t in [0, 0.1), kappa 1):
 * Over a run: at t = 0.025 (39 coarse RK3 steps) the corrected FP32 256^2 solver's relative L1
@@ -88,9 +90,9 @@ def test_early_network_run_level_gain():
 
 def test_finetuned_network_run_level_gain():
     """The a posteriori fine-tuned network on a held-out seed: every conserved variable's L1 against
-    the restricted 1024^2 run below the plain FP32 solver's at t = 0.05 (by at least 5 %, 20 % on
-    average) and at t = 0.1 (by at least 0.5 %, 5 % on average; measured on seed 1002: 8 / 24 / 32 /
-    31 % and 2.1 / 7.6 / 9.6 / 13 % for mass, m_x, m_y, E)."""
+    the restricted 1024^2 run below the plain FP32 solver's at t = 0.05 (by at least 15 %, 30 % on
+    average) and at t = 0.1 (by at least 5 %, 12 % on average; measured on seed 1002: 28 / 30 / 39 /
+    44 % and 9 / 15 / 17 / 22 % for mass, m_x, m_y, E)."""
     from gaukuk_amd import hip
     net = hip.Net.load(NET_ES)
     with np.load(NET_ES) as z:
@@ -98,7 +100,7 @@ def test_finetuned_network_run_level_gain():
     r = learned.evaluate(net, seeds=(1002,), times=(0.05, 0.1), kappas=(net.kappa,), log=lambda m: None)["seeds"]["1002"]
     key = f"learned_fp32_k{net.kappa:g}_256"
     assert r[key]["reached"]
-    for k, (lo, mean) in enumerate(((0.05, 0.20), (0.005, 0.05))):
+    for k, (lo, mean) in enumerate(((0.15, 0.30), (0.05, 0.12))):
         gain = 1 - np.array(r[key]["l1"][k]) / np.array(r["fp32_256"]["l1"][k])
         print(f"fine-tuned network gain at t = {(0.05, 0.1)[k]} (mass, m_x, m_y, E):", gain, flush=True)
         assert np.all(gain >= lo), gain
