@@ -2,11 +2,11 @@
 (training used seeds 0..63).
 
 gaukuk_amd/data/kh256_flux_mlp_es.npz (the early network fine-tuned a posteriori by evolution
-strategies on whole-run errors at t = 0.025, 0.05, 0.1, seeds 0..3, 1680 iterations;
+strategies on whole-run errors at t = 0.025, 0.05, 0.1, seeds 0..3, 2580 iterations;
 gaukuk_amd.learned.finetune):
-* Over a run on the held-out seeds 1000-1002 the gain lasts: 29-31 % (mass), 33-35 % (m_x),
-  35-36 % (m_y), 44-45 % (E) at t = 0.025; 26-30, 29-30, 39-41, 42-44 % at t = 0.05; 8-9, 10-16,
-  15-18, 21-22 % at t = 0.1.
+* Over a run on the held-out seeds 1000-1002 the gain lasts: 33-35 % (mass), 37-38 % (m_x),
+  35-36 % (m_y), 44 % (E) at t = 0.025; 28-32, 32-33, 40-42, 43-45 % at t = 0.05; 8-10, 11-19,
+  16-20, 22-24 % at t = 0.1.
 
 gaukuk_amd/data/kh256_flux_mlp_early.npz (trained on frames of t in [0, 0.1), kappa 1):
 * Over a run: at t = 0.025 (39 coarse RK3 steps) the corrected FP32 256^2 solver's relative L1
@@ -91,8 +91,8 @@ def test_early_network_run_level_gain():
 def test_finetuned_network_run_level_gain():
     """The a posteriori fine-tuned network on a held-out seed: every conserved variable's L1 against
     the restricted 1024^2 run below the plain FP32 solver's at t = 0.05 (by at least 15 %, 30 % on
-    average) and at t = 0.1 (by at least 5 %, 12 % on average; measured on seed 1002: 28 / 30 / 39 /
-    44 % and 9 / 15 / 17 / 22 % for mass, m_x, m_y, E)."""
+    average) and at t = 0.1 (by at least 5 %, 12 % on average; measured on seed 1002: 30 / 33 / 40 /
+    44 % and 10 / 17 / 19 / 24 % for mass, m_x, m_y, E)."""
     from gaukuk_amd import hip
     net = hip.Net.load(NET_ES)
     with np.load(NET_ES) as z:
