@@ -595,8 +595,9 @@ __global__ __launch_bounds__(256) void face_targets_kernel(const double* facc, i
 // ---------------------------------------------------------------------------------------------
 // The corrected FP32 stage: one 512-thread workgroup per 16 x 8 tile of cells.
 //   1. the tile and a 2-cell halo -> LDS (conserved after the boundary rules, and primitives)
-//   2. PLM + HLLC at the tile's 136 x faces and 144 y faces -> LDS, one face per thread
-//   3. the MLP on 18 tiles of 16 faces (BF16 MFMA), sensor and positivity tests, correction
+//   2. PLM + HLLC at the tile's 136 x faces and 144 y faces -> LDS, one face per thread, and from
+//      the same four cells each face's network inputs, dissipation bound and sensor
+//   3. the MLP on 18 tiles of 16 faces (BF16 MFMA), positivity tests, correction
 //   4. flux divergence, RK combine, floors, store, max wave speed
 // A tile this size needs 53 KB of LDS and 128 registers per lane, so two workgroups (16 waves)
 // share a CU.  (A 16 x 16 tile needed 88 KB, one workgroup per CU, and 544 faces for 512 threads:
@@ -699,7 +700,10 @@ void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
         if (f < kLFx) { const int fy = f / (kLTX + 1), fx = f - fy * (kLTX + 1); cy = fy + kHalo; cx = fx + k; }
         else { const int f2 = f - kLFx, fy = f2 / kLTX, fx = f2 - fy * kLTX; cy = fy + k; cx = fx + kHalo; }
     };
+    // One face per thread: PLM + HLLC flux and, with a network, from the same four cells the face
+    // frame, features, dissipation bound and sensor (M1)
     for (int f = tid; f < kLFaces; f += kLThreads) {
+        const bool yface = f >= kLFx;
         float w[4][5], F[5];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -708,25 +712,17 @@ void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
 #pragma unroll
             for (int n = 0; n < 5; n++) w[k][n] = sw[n][cy][cx];
         }
-        face_flux(a.eos, w, f >= kLFx, F);
+        face_flux(a.eos, w, yface, F);
 #pragma unroll
         for (int n = 0; n < 5; n++) sf[n][f] = F[n];
-    }
-    __syncthreads();
-
-    if (net.enabled) {
-        // M1, one face per thread: frame, features, dissipation bound, sensor
-        for (int f = tid; f < kLFaces; f += kLThreads) {
-            const bool yface = f >= kLFx;
+        if (net.enabled) {
             float st[4][4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                int cy, cx;
-                face_cell(f, k, cy, cx);
-                st[k][0] = sw[0][cy][cx];
-                st[k][1] = yface ? sw[2][cy][cx] : sw[1][cy][cx];
-                st[k][2] = yface ? sw[1][cy][cx] : sw[2][cy][cx];
-                st[k][3] = sw[4][cy][cx];
+                st[k][0] = w[k][0];
+                st[k][1] = yface ? w[k][2] : w[k][1];
+                st[k][2] = yface ? w[k][1] : w[k][2];
+                st[k][3] = w[k][4];
             }
             FaceFrame<float> fr;
             fr.init(st, gamma);
@@ -744,7 +740,10 @@ void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
             *reinterpret_cast<float4*>(&smeta[f][0]) = make_float4(bd[0], bd[1], bd[2], bd[3]);
             *reinterpret_cast<float4*>(&smeta[f][4]) = make_float4(fr.rb, fr.cb, fr.vtb, sensor_fires(p) ? 1.f : 0.f);
         }
-        __syncthreads();
+    }
+    __syncthreads();
+
+    if (net.enabled) {
         // M2, per wave: the network on tiles of 16 faces (lane (g, s) feeds inputs 4g..4g+3 of face s)
         {
             mlp::Regs wr;
