@@ -808,13 +808,13 @@ void learned_stage_kernel(const StageArgs<float> a, const NetArgs net) {
                 else n_pos++;
             }
         }
-        // face counts: wave sums to LDS; after the barrier one thread adds them to this tile's
-        // slot (plain read-modify-write: a tile belongs to one workgroup, launches are ordered)
-        for (int m = 1; m < 64; m <<= 1) {
-            n_corr += __shfl_xor(n_corr, m);
-            n_sensor += __shfl_xor(n_sensor, m);
-            n_pos += __shfl_xor(n_pos, m);
-        }
+        // face counts: wave sums to LDS (a thread holds at most one face, so a wave's count is
+        // the population of a ballot); after the barrier one thread adds them to this tile's slot
+        // (plain read-modify-write: a tile belongs to one workgroup, launches are ordered)
+        static_assert(kLFaces <= kLThreads, "one face per thread");
+        n_corr = __popcll(__ballot(n_corr != 0));
+        n_sensor = __popcll(__ballot(n_sensor != 0));
+        n_pos = __popcll(__ballot(n_pos != 0));
         if (lane == 0) { scount[wv][0] = n_corr; scount[wv][1] = n_sensor; scount[wv][2] = n_pos; }
         __syncthreads();
         if (net.stats && tid == 0) {
