@@ -601,7 +601,7 @@ __global__ __launch_bounds__(256) void face_targets_kernel(const double* facc, i
 //   4. flux divergence, RK combine, floors, store, max wave speed
 // A tile this size needs 53 KB of LDS and 128 registers per lane, so two workgroups (16 waves)
 // share a CU.  (A 16 x 16 tile needed 88 KB, one workgroup per CU, and 544 faces for 512 threads:
-// two rounds of every face phase.  At 256^2 the two shapes measured the same, 17-18 us per launch:
+// two rounds of every face phase.  At 256^2 the two shapes measured the same per launch:
 // the phases are latency chains separated by barriers, not issue-bound.)  Every face is solved
 // from the same inputs whatever tile holds it, and the network treats each face (one MFMA column)
 // independently, so results do not depend on the tile shape.
