@@ -281,6 +281,21 @@ def one_step_error(net, seed: int = 1000, t0: float = 0.3, n_fine: int = 1024, n
     return {"seed": seed, "t0": t0, "dt": D, "plain": err(up), "corrected": err(uc)}
 
 
+def continued_record(meta: dict, hist, iters: int) -> dict:
+    """The metadata to save after `iters` fine-tuning iterations with objective history `hist`
+    ([iteration, objective] pairs) on a network whose saved metadata is `meta`.  A network
+    fine-tuned before continues that record: the iterations add up and the histories are joined
+    (this run's iterations offset by the earlier total); the earlier run's other ft_* settings are
+    dropped, for the caller to replace with this run's."""
+    prev_iters = int(meta.get("ft_iters", 0))
+    prev_hist = np.asarray(meta.get("ft_history", np.zeros((0, 2))), dtype=np.float64).reshape(-1, 2)
+    out = {k: v for k, v in meta.items() if not k.startswith("ft_")}
+    new = np.array([[prev_iters + h[0], h[1]] for h in hist], dtype=np.float64).reshape(-1, 2)
+    out["ft_iters"] = prev_iters + int(iters)
+    out["ft_history"] = np.concatenate([prev_hist, new])
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -334,14 +349,9 @@ def main(argv=None) -> int:
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         with np.load(a.net, allow_pickle=False) as z:
             meta = {k: z[k] for k in z.files if k not in ("params", "in_scale", "out_scale", "kappa")}
-        # a network fine-tuned before continues that record: iterations add up and the histories
-        # are joined (the earlier run's settings are replaced by this run's)
-        prev_iters = int(meta.get("ft_iters", 0))
-        prev_hist = np.asarray(meta.get("ft_history", np.zeros((0, 2))), dtype=np.float64).reshape(-1, 2)
-        meta = {k: v for k, v in meta.items() if not k.startswith("ft_")}
-        hist_all = np.concatenate([prev_hist, np.array([[prev_iters + h[0], h[1]] for h in hist], dtype=np.float64)])
+        meta = continued_record(meta, hist, a.iters)
         net.save(a.out, **meta, ft_seeds=np.array(ft_seeds), ft_times=np.array(ft_times),
-                 ft_iters=prev_iters + a.iters, ft_pairs=a.pairs, ft_sigma=a.sigma, ft_lr=a.lr, ft_history=hist_all)
+                 ft_pairs=a.pairs, ft_sigma=a.sigma, ft_lr=a.lr)
         log(f"finetune {time.time() - t0:.1f} s, saved {a.out}")
         rep = {"finetune_s": time.time() - t0, "history": hist, "eval": evaluate(net, seeds, times, kappas, log=log)}
     elif a.cmd == "train":

@@ -206,3 +206,4 @@ def test_dissipation_bound():
     bac = L.dissipation_bound(L.face_features(ac, 1.4), 1.4)[0]
     assert bac[0] > 0 and bac[1] > 0 and bac[3] > 0
     assert bac[2] == 0                              # no tangential velocity: |r_mt| = |v_t| = 0
+
